@@ -1,0 +1,172 @@
+/*
+ * kwave_hip.h -- C ABI of the MI355X (gfx950) k-space pseudospectral solver.
+ *
+ * The reference reaches its accelerated time loop through a process
+ * boundary: kspaceFirstOrder3DC.m:536-566 runs
+ *   system('... ./kspaceFirstOrder-{OMP,CUDA} -i in.h5 -o out.h5 <flags>')
+ * after kspaceFirstOrder_saveToDisk.m writes the input file, and reads the
+ * results back with h5read (kspaceFirstOrder3DC.m:568-778).  This header is
+ * the in-process form of that contract: a simulation is described by the
+ * datasets of that input file (as plain host arrays), run, and its outputs
+ * fetched by the dataset names of the output file.  The file-level form of
+ * the same contract (HDF5 in, HDF5 out) is in kwave_io.h and the
+ * kspaceFirstOrder-HIP executable built on it.
+ *
+ * Conventions: status codes, never exceptions; host buffers are caller-owned
+ * and copied in; device buffers are library-owned; handles are independent
+ * (one HIP stream each); kw_last_error() returns a thread-local message.
+ * Arrays are in HDF5 C order: a grid field is (Nz, Ny, Nx) with x fastest,
+ * which is MATLAB's column-major [Nx, Ny, Nz].
+ */
+#ifndef KWAVE_HIP_H
+#define KWAVE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KW_ABI_VERSION 1
+
+typedef enum kw_status {
+  KW_OK = 0,
+  KW_ERR_INVALID = 1,     /* malformed description or argument */
+  KW_ERR_UNSUPPORTED = 2, /* valid k-Wave input this build does not run */
+  KW_ERR_DEVICE = 3,      /* HIP runtime error */
+  KW_ERR_OOM = 4,         /* device allocation failed */
+  KW_ERR_IO = 5           /* file I/O (kwave_io.h) */
+} kw_status;
+
+/* A float32 dataset that is a scalar (count == 1) or a full grid field
+ * (count == Nx*Ny*Nz).  Mirrors writeMatrix.m's scalar-or-matrix datasets
+ * (c0, rho0, rho0_sgx/y/z, BonA, alpha_coeff: saveToDisk.m:62-73, 155-215). */
+typedef struct kw_real_array {
+  const float* data;
+  uint64_t count;
+} kw_real_array;
+
+/* Output selection: the binary's --p_raw ... --u_final flags
+ * (kspaceFirstOrder3DC.m:466-519). */
+enum kw_record_flag {
+  KW_REC_P = 1u << 0,
+  KW_REC_P_MAX = 1u << 1,
+  KW_REC_P_MIN = 1u << 2,
+  KW_REC_P_RMS = 1u << 3,
+  KW_REC_P_MAX_ALL = 1u << 4,
+  KW_REC_P_MIN_ALL = 1u << 5,
+  KW_REC_P_FINAL = 1u << 6,
+  KW_REC_U = 1u << 7,
+  KW_REC_U_MAX = 1u << 8,
+  KW_REC_U_MIN = 1u << 9,
+  KW_REC_U_RMS = 1u << 10,
+  KW_REC_U_MAX_ALL = 1u << 11,
+  KW_REC_U_MIN_ALL = 1u << 12,
+  KW_REC_U_FINAL = 1u << 13
+};
+
+/* Output datasets (names of the output file read at kspaceFirstOrder3DC.m:568-778).
+ * Velocity outputs come in x, y, z order: KW_OUT_UX + axis, etc. */
+typedef enum kw_output_id {
+  KW_OUT_P = 0,         /* (Nt_rec, Ns)  float32 */
+  KW_OUT_P_MAX = 1,     /* (Ns) */
+  KW_OUT_P_MIN = 2,
+  KW_OUT_P_RMS = 3,
+  KW_OUT_P_MAX_ALL = 4, /* (Nz, Ny, Nx) */
+  KW_OUT_P_MIN_ALL = 5,
+  KW_OUT_P_FINAL = 6,
+  KW_OUT_UX = 7, /* +axis: (Nt_rec, Ns) */
+  KW_OUT_UX_MAX = 10,
+  KW_OUT_UX_MIN = 13,
+  KW_OUT_UX_RMS = 16,
+  KW_OUT_UX_MAX_ALL = 19,
+  KW_OUT_UX_MIN_ALL = 22,
+  KW_OUT_UX_FINAL = 25,
+  KW_OUT_COUNT = 28
+} kw_output_id;
+
+/* One simulation: the dataset set of the HDF5 input file
+ * (kspaceFirstOrder_saveToDisk.m:44-414, writeFlags.m, writeGrid.m).
+ * 2-D inputs have Nz == 1 and no z datasets (saveToDisk.m:420-427).
+ * Indices are 1-based linear indices as written by the front end (find()). */
+typedef struct kw_sim_desc {
+  uint32_t abi_version; /* KW_ABI_VERSION */
+  int32_t device;       /* HIP device ordinal (the binary's -g) */
+  uint64_t Nx, Ny, Nz, Nt;
+  double dx, dy, dz, dt, c_ref;
+  uint64_t pml_x_size, pml_y_size, pml_z_size;
+  double pml_x_alpha, pml_y_alpha, pml_z_alpha;
+  kw_real_array c0, rho0, rho0_sgx, rho0_sgy, rho0_sgz;
+  uint64_t nonlinear_flag;
+  kw_real_array BonA;
+  uint64_t absorbing_flag; /* 0 lossless, 1 power law, 2 stokes */
+  kw_real_array alpha_coeff; /* dB/(MHz^y cm), converted by db2neper.m:50 inside */
+  double alpha_power;
+  /* initial pressure (p0_source_flag / p0_source_input); NULL if none */
+  const float* p0_source_input;
+  /* pressure source: flag = number of signal samples (0 = none) */
+  uint64_t p_source_flag, p_source_mode, p_source_many, p_source_npoints;
+  const uint64_t* p_source_index; /* p_source_npoints */
+  const float* p_source_input;    /* (p_source_flag, many ? npoints : 1) */
+  /* velocity sources share u_source_index */
+  uint64_t ux_source_flag, uy_source_flag, uz_source_flag;
+  uint64_t u_source_mode, u_source_many, u_source_npoints;
+  const uint64_t* u_source_index;
+  const float* ux_source_input;
+  const float* uy_source_input;
+  const float* uz_source_input;
+  /* sensor (sensor_mask_type 0: binary-mask indices) */
+  uint64_t sensor_npoints;
+  const uint64_t* sensor_mask_index;
+  /* outputs */
+  uint64_t record_flags;       /* kw_record_flag bits */
+  uint64_t record_start_index; /* the binary's -s, 1-based (0 treated as 1) */
+} kw_sim_desc;
+
+typedef struct kw_sim kw_sim;
+
+/* Copies the description to the device, runs the FP64 operator precompute
+ * (kspaceFirstOrder3D.m:583-659) and allocates state.
+ * Replaces: the binary's start-up after system() at kspaceFirstOrder3DC.m:544. */
+kw_status kw_sim_create(const kw_sim_desc* desc, kw_sim** out);
+
+/* Advances nsteps time steps (clamped to Nt) on `stream` (a hipStream_t;
+ * NULL uses the handle's own stream).  Asynchronous with respect to the host.
+ * Replaces: the time loop inside the binary (kspaceFirstOrder3D.m:767-1206). */
+kw_status kw_sim_run(kw_sim* sim, uint64_t nsteps, void* stream);
+
+/* Blocks until all work queued on the handle's streams has finished. */
+kw_status kw_sim_sync(kw_sim* sim);
+
+/* Number of time steps completed so far. */
+uint64_t kw_sim_steps_done(const kw_sim* sim);
+
+/* Byte size of an output (0 if not recorded). */
+size_t kw_sim_output_bytes(const kw_sim* sim, int output_id);
+
+/* Copies an output to host memory (synchronises first).
+ * Replaces: h5read of the output file (kspaceFirstOrder3DC.m:568-778). */
+kw_status kw_sim_fetch(kw_sim* sim, int output_id, void* host_dst, size_t bytes);
+
+/* Frees device memory and the handle. */
+kw_status kw_sim_destroy(kw_sim* sim);
+
+/* Thread-local description of the last failure on this thread. */
+const char* kw_last_error(void);
+
+/* Per-kernel timing: when enabled, every launch of kw_sim_run is bracketed
+ * by HIP events on its stream and its duration accumulated per kernel kind.
+ * kw_sim_profile_get returns the number of kinds and fills up to `cap`
+ * entries of names (static strings), total milliseconds and launch counts. */
+kw_status kw_sim_set_profiling(kw_sim* sim, int enable);
+int kw_sim_profile_get(kw_sim* sim, const char** names, double* total_ms, uint64_t* launches, int cap);
+
+/* Device memory (bytes) a description would need; 0 on invalid input.
+ * Used for HBM-footprint admission by the serving loop. */
+uint64_t kw_sim_footprint_bytes(const kw_sim_desc* desc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KWAVE_HIP_H */

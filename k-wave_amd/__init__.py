@@ -1,0 +1,17 @@
+"""MI355X-native k-space pseudospectral time loop (k-Wave kspaceFirstOrder2D/3D).
+
+Imported as ``kwave_amd`` (the directory name carries a hyphen; entry points
+register it under that name).  Layout:
+
+* ``csrc/``       HIP kernels for gfx950 + C++ host orchestration + C ABI
+* ``native``      ctypes binding of include/kwave_hip.h (Simulation, run)
+* ``frontend``    k-Wave style inputs -> input-file dataset dict
+* ``geometry``    makeDisc/makeBall/makeCartCircle/toneBurst/smooth
+* ``h5format``    HDF5 input/output files (libkwave_io.so)
+* ``build``       in-tree build recipe
+"""
+
+from . import frontend, geometry  # noqa: F401
+from .native import Simulation, footprint_bytes, load_library, run  # noqa: F401
+
+__all__ = ["Simulation", "run", "footprint_bytes", "load_library", "frontend", "geometry"]

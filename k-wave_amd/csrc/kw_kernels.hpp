@@ -1,0 +1,139 @@
+// Argument blocks shared by the HIP kernels (kw_kernels.hip) and the host
+// orchestration (kw_sim.cpp).  Plain structs passed by value as kernel
+// arguments.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kw_fft.hpp"
+
+namespace kw {
+
+// Grid geometry.  Real fields are C-order (Nz, Ny, Nx): idx = x + Nx*(y + Ny*z).
+// x-spectra ("X domain", after the x R2C) are complex (Nz, Ny, Nxh), Nxh = Nx/2+1.
+struct Grid {
+  int Nx, Ny, Nz, Nxh, dim;
+  int rows;  // Ny * Nz
+  long long N;
+};
+
+// Scalar-or-field value: field pointer (nullptr for a scalar) + scalar.
+struct SF {
+  const float* f;
+  float s;
+  __device__ __forceinline__ float at(long long i) const { return f ? f[i] : s; }
+};
+
+struct Medium {
+  SF c0, rho0, rsg[3], bona, tau, eta;
+  const float* pml[3];     // non-staggered PML, length N_axis
+  const float* pml_sg[3];  // staggered PML
+  float dt;
+  int nonlinear;
+  int absorbing;  // 0 lossless, 1 power law, 2 stokes
+};
+
+// Source for one quantity (a velocity component, or the pressure).
+struct Src {
+  int active;        // signal still running at this step
+  int mode;          // 0 dirichlet, 1 additive-no-correction, 2 additive
+  int many;          // one signal per point
+  const float* sig;  // (Nt_src, nseries) for this step's row already offset
+  const int* row_ptr;       // CSR over rows (rows+1), source points sorted by index
+  const long long* lin;     // 0-based linear index of each source point
+  const float* pattern;  // additive, single signal: corrected unit pattern (N)
+  const float2* spec;    // additive, many signals: X-domain corrected source
+  float sval;            // additive single signal: value at this step
+};
+
+// Sensor recording for one quantity.
+struct Sens {
+  int record;   // record at this step
+  int first;    // first recorded step (initialise max/min)
+  long long t_rec;
+  int ns;
+  const int* row_ptr;    // CSR over rows (rows+1), sensor points sorted by index
+  const long long* lin;  // 0-based linear index of each sensor point
+  float* raw;       // (Nt_rec, ns)
+  float* vmax;
+  float* vmin;
+  double* rms_sum;  // compensated sum of squares (sum, compensation pairs)
+  float* max_all;
+  float* min_all;
+  float* final_field;  // write the field (p_final) when non-null
+};
+
+enum SpecOp { OP_SCALE = 0, OP_DPOS_K = 1, OP_DNEG_K = 2, OP_NABLA1 = 3, OP_NABLA2 = 4, OP_SKAPPA = 5 };
+
+struct SpecOut {
+  int src;   // input index
+  int op;    // SpecOp
+  int axis;  // derivative axis for OP_DPOS_K / OP_DNEG_K
+};
+
+struct SpecTables {
+  const float2* dpos[3];
+  const float2* dneg[3];
+  const float* kappa;
+  const float* skappa;
+  const float* nabla1;
+  const float* nabla2;
+};
+
+struct LastArgs {
+  float2* in[3];
+  float2* out[3];
+  int nin, nout;
+  SpecOut outs[3];
+  float scale;
+};
+
+// Host-side launch wrappers (kw_kernels.hip).
+struct XLaunch {
+  int pairs;    // packed row pairs per workgroup
+  int threads;  // workgroup size
+};
+struct TileLaunch {
+  int tx;       // pencils per workgroup
+  int threads;
+};
+
+void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, float2* const* f,
+                  int nf, float sign, TileLaunch tl);
+void launch_last(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const LastArgs& a,
+                 const SpecTables& T, TileLaunch tl);
+void launch_x_velocity(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
+                       float2* const* G, float* const* u, const Src* src, const Sens* sens, int init,
+                       XLaunch xl);
+void launch_x_density(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
+                      float2* const* G, float* const* rho, const Src& psrc, float* srho_out, float2* pk,
+                      const Sens& ps, XLaunch xl);
+void launch_x_pressure(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
+                       const float2* GA, const float2* GB, const float* srho, float2* pk, const Sens& ps,
+                       XLaunch xl);
+void launch_x_init(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
+                   const float* p0, float* const* rho, float2* pk, const Sens& ps, XLaunch xl);
+void launch_x_source(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Src& src,
+                     int unit, float2* out, XLaunch xl);
+void launch_x_c2r(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const float2* in,
+                  float* out, XLaunch xl);
+
+// FP64 precompute
+struct OpParams {
+  double dx[3];
+  double dt, c_ref;
+  double alpha_power;
+  int absorbing;
+};
+void launch_twiddles(hipStream_t s, float2* tw, int n);
+void launch_dvec(hipStream_t s, float2* dpos, float2* dneg, int nfull, int nout, double d);
+void launch_pml(hipStream_t s, float* pml, int n, double d, double dt, double c, int size, double alpha,
+                int staggered);
+void launch_spectral_tables(hipStream_t s, const Grid& g, const OpParams& p, float* kappa, float* skappa,
+                            float* nabla1, float* nabla2);
+void launch_absorb_coeffs(hipStream_t s, long long n, const float* c0, float c0s, const float* alpha,
+                          float alphas, double y, int absorbing, float* tau, float* eta);
+void launch_reciprocal(hipStream_t s, long long n, const float* in, float* out);
+void launch_finish_rms(hipStream_t s, long long n, const double* sum, double count, float* out);
+
+}  // namespace kw

@@ -1,0 +1,935 @@
+// Host orchestration behind the C ABI of include/kwave_hip.h: validates the
+// input-file dataset set, builds FFT plans and launch geometry, runs the
+// FP64 operator precompute on the device, and sequences the per-step passes
+// of kw_kernels.hip.  Statement references are to k-Wave/kspaceFirstOrder3D.m.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kwave_hip.h"
+#include "kw_kernels.hpp"
+
+using namespace kw;
+
+namespace {
+
+thread_local std::string g_err;
+
+kw_status fail(kw_status st, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define HIPCHK(x)                                   \
+  do {                                              \
+    hipError_t _e = (x);                            \
+    if (_e != hipSuccess) throw HipError{_e, #x};   \
+  } while (0)
+
+struct OomError {
+  size_t bytes;
+};
+
+// Factor n into radices {8,4,2,3,5,7}; false if another prime divides n.
+bool make_axis(int n, FftAxis* ax) {
+  ax->n = n;
+  ax->nstages = 0;
+  int m = n;
+  auto push = [&](int r) {
+    if (ax->nstages >= KW_MAX_STAGES) return false;
+    ax->radix[ax->nstages++] = r;
+    m /= r;
+    return true;
+  };
+  while (m % 8 == 0 && m >= 8)
+    if (!push(8)) return false;
+  while (m % 4 == 0)
+    if (!push(4)) return false;
+  while (m % 2 == 0)
+    if (!push(2)) return false;
+  for (int r : {3, 5, 7})
+    while (m % r == 0)
+      if (!push(r)) return false;
+  if (ax->nstages == 0) {  // n == 1: identity transform
+    ax->radix[0] = 1;
+  }
+  return m == 1;
+}
+
+enum KernelKind { KK_YPASS, KK_LAST, KK_XVEL, KK_XDENS, KK_XPRES, KK_XINIT, KK_XSRC, KK_COUNT };
+const char* kKindNames[KK_COUNT] = {"y_pass", "last_axis", "x_velocity", "x_density",
+                                    "x_pressure", "x_init", "x_source"};
+
+struct SrcData {
+  uint64_t flag = 0;  // signal length
+  int mode = 0, many = 0;
+  uint64_t nseries = 0, npoints = 0;
+  float* sig = nullptr;          // device (flag, nseries)
+  int* row_ptr = nullptr;        // device (rows+1)
+  long long* lin = nullptr;      // device (npoints)
+  float* pattern = nullptr;      // device N (additive, single signal)
+  float2* spec = nullptr;        // device X-domain (additive, many)
+  std::vector<float> host_sig;   // for the single-signal value per step
+};
+
+}  // namespace
+
+struct kw_sim {
+  Grid g{};
+  int device = 0;
+  long long Nt = 0, t = 0, start = 1;
+  uint64_t rec = 0;
+  hipStream_t stream = nullptr;
+  FftAxis axX{}, axY{}, axL{};
+  float2 *twX = nullptr, *twY = nullptr, *twL = nullptr;
+  XLaunch xl{};
+  TileLaunch tly{}, tll{};
+  // state
+  float* u[3] = {};
+  float* rho[3] = {};
+  float* srho = nullptr;
+  float2* Pk = nullptr;
+  float2* G[3] = {};
+  float* p0 = nullptr;
+  // medium / operators
+  Medium m{};
+  SpecTables T{};
+  std::vector<void*> owned;
+  SrcData psrc, usrc[3];
+  uint64_t u_flags[3] = {};
+  // sensor
+  int ns = 0;
+  int* s_row_ptr = nullptr;
+  long long* s_lin = nullptr;
+  float *p_raw = nullptr, *p_max = nullptr, *p_min = nullptr, *p_max_all = nullptr, *p_min_all = nullptr,
+        *p_final = nullptr;
+  double* p_rms = nullptr;
+  float *u_raw[3] = {}, *u_max[3] = {}, *u_min[3] = {}, *u_max_all[3] = {}, *u_min_all[3] = {};
+  double* u_rms[3] = {};
+  long long nrec_total = 0;  // Nt - start + 1
+  // profiling
+  bool profiling = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double prof_ms[KK_COUNT] = {};
+  uint64_t prof_n[KK_COUNT] = {};
+
+  template <typename T_>
+  T_* alloc(size_t count) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T_), 16);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw OomError{bytes};
+    }
+    owned.push_back(p);
+    return static_cast<T_*>(p);
+  }
+  template <typename T_>
+  T_* upload(const T_* host, size_t count) {
+    T_* d = alloc<T_>(count);
+    HIPCHK(hipMemcpy(d, host, count * sizeof(T_), hipMemcpyHostToDevice));
+    return d;
+  }
+  void zero(void* p, size_t bytes) { HIPCHK(hipMemsetAsync(p, 0, bytes, stream)); }
+
+  long long nspec() const { return (long long)g.Nxh * g.Ny * g.Nz; }
+};
+
+namespace {
+
+void free_all(kw_sim* s) {
+  for (void* p : s->owned) (void)hipFree(p);
+  s->owned.clear();
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+// Launch geometry for the x kernels: P row pairs per workgroup with
+// P*Nx <= KW_MAXE * threads, aiming at >= 1024 workgroups.
+XLaunch choose_x(const Grid& g) {
+  const int maxe = 4;
+  int pmax = std::max(1, (maxe * 256) / g.Nx);
+  int p = std::max(1, std::min(pmax, g.rows / 2048));
+  p = std::min(p, std::max(1, (g.rows + 1) / 2));
+  int threads = ((p * g.Nx / 2 + 63) / 64) * 64;
+  threads = std::max(64, std::min(256, threads));
+  while (p * g.Nx > maxe * threads && threads < 256) threads += 64;
+  return XLaunch{p, threads};
+}
+
+// Pencil tile width: split `width` columns evenly into tiles of <= 16 that
+// keep `nbuf` LDS buffers of n*tx complex values within 64 KiB.
+TileLaunch choose_tile(int width, int n, int nbuf) {
+  const size_t cap = 65536;
+  int txmax = 16;
+  while (txmax > 1 && (size_t)n * sizeof(float2) * (1 + (size_t)nbuf * txmax) > cap) --txmax;
+  int ntiles = (width + txmax - 1) / txmax;
+  int tx = (width + ntiles - 1) / ntiles;
+  return TileLaunch{tx, 256};
+}
+
+size_t x_smem_bytes(const Grid& g, int P, int nbuf, bool slot) {
+  size_t s = (size_t)g.Nx * sizeof(float2) + (size_t)nbuf * P * g.Nx * sizeof(float2);
+  if (slot) s += (size_t)2 * P * g.Nx * sizeof(int);
+  return s;
+}
+
+// CSR of sorted 0-based linear indices over rows of length Nx.
+void build_rows(const std::vector<long long>& lin, const Grid& g, std::vector<int>& row_ptr) {
+  row_ptr.assign(g.rows + 1, 0);
+  size_t j = 0;
+  for (int r = 0; r <= g.rows; ++r) {
+    const long long lim = (long long)r * g.Nx;
+    while (j < lin.size() && lin[j] < lim) ++j;
+    row_ptr[r] = (int)j;
+  }
+}
+
+kw_status load_indices(const uint64_t* idx, uint64_t n, long long N, const char* what,
+                       std::vector<long long>& out) {
+  out.resize(n);
+  for (uint64_t j = 0; j < n; ++j) {
+    if (idx[j] < 1 || (long long)idx[j] > N)
+      return fail(KW_ERR_INVALID, "%s[%llu] = %llu outside 1..%lld", what, (unsigned long long)j,
+                  (unsigned long long)idx[j], N);
+    out[j] = (long long)idx[j] - 1;
+    if (j > 0 && out[j] <= out[j - 1])
+      return fail(KW_ERR_UNSUPPORTED, "%s must be strictly increasing (binary-mask find() order)", what);
+  }
+  return KW_OK;
+}
+
+SF make_sf(kw_sim* s, const kw_real_array& a) {
+  SF r{nullptr, 0.f};
+  if (a.count == 1) {
+    r.s = a.data[0];
+  } else {
+    r.f = s->upload(a.data, (size_t)a.count);
+  }
+  return r;
+}
+
+kw_status check_array(const kw_real_array& a, long long N, const char* name, bool required) {
+  if (!a.data || a.count == 0) {
+    if (required) return fail(KW_ERR_INVALID, "%s missing", name);
+    return KW_OK;
+  }
+  if (a.count != 1 && (long long)a.count != N)
+    return fail(KW_ERR_INVALID, "%s has %llu values; expected 1 or %lld", name, (unsigned long long)a.count, N);
+  return KW_OK;
+}
+
+// ---- per-kernel timing ------------------------------------------------------
+struct Prof {
+  kw_sim* s;
+  hipStream_t st;
+  int kind;
+  Prof(kw_sim* s_, hipStream_t st_, int k) : s(s_), st(st_), kind(k) {
+    if (s->profiling) HIPCHK(hipEventRecord(s->ev0, st));
+  }
+  ~Prof() noexcept(false) {
+    if (s->profiling) {
+      HIPCHK(hipEventRecord(s->ev1, st));
+      HIPCHK(hipEventSynchronize(s->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      s->prof_ms[kind] += ms;
+      s->prof_n[kind] += 1;
+    }
+  }
+};
+
+// ---- transform helpers ------------------------------------------------------
+void ypass(kw_sim* s, hipStream_t st, float2* const* f, int nf, float sign) {
+  if (s->g.dim != 3) return;
+  Prof pr(s, st, KK_YPASS);
+  launch_ypass(st, s->g, s->axY, s->twY, f, nf, sign, s->tly);
+}
+
+void last(kw_sim* s, hipStream_t st, const LastArgs& a) {
+  Prof pr(s, st, KK_LAST);
+  launch_last(st, s->g, s->axL, s->twL, a, s->T, s->tll);
+}
+
+// Corrected additive source: real(ifft(source_kappa .* fft(scatter))) in X
+// domain, (3D.m:815-822); unit scatters 1.0 (pattern build).
+void source_spectrum(kw_sim* s, hipStream_t st, const Src& src, int unit, float2* out) {
+  {
+    Prof pr(s, st, KK_XSRC);
+    launch_x_source(st, s->g, s->axX, s->twX, src, unit, out, s->xl);
+  }
+  float2* f[1] = {out};
+  ypass(s, st, f, 1, -1.f);
+  LastArgs a{};
+  a.in[0] = out;
+  a.out[0] = out;
+  a.nin = 1;
+  a.nout = 1;
+  a.outs[0] = SpecOut{0, OP_SKAPPA, 0};
+  a.scale = (float)(1.0 / (double)s->g.N);
+  last(s, st, a);
+  ypass(s, st, f, 1, 1.f);
+}
+
+Src make_src(const kw_sim* s, const SrcData& d, long long t) {
+  Src r{};
+  r.active = d.flag >= (uint64_t)t;
+  r.mode = d.mode;
+  r.many = d.many;
+  r.row_ptr = d.row_ptr;
+  r.lin = d.lin;
+  r.pattern = d.pattern;
+  r.spec = d.spec;
+  if (r.active) {
+    r.sig = d.sig + (size_t)(t - 1) * d.nseries;
+    r.sval = d.host_sig.empty() ? 0.f : d.host_sig[(size_t)(t - 1) * d.nseries];
+  }
+  (void)s;
+  return r;
+}
+
+Sens make_sens_p(const kw_sim* s, long long t, bool final_step) {
+  Sens r{};
+  const bool rec = t >= s->start;
+  r.record = rec;
+  r.first = rec && (t == s->start);
+  r.t_rec = rec ? t - s->start : 0;
+  r.ns = s->ns;
+  r.row_ptr = s->s_row_ptr;
+  r.lin = s->s_lin;
+  r.raw = s->p_raw;
+  r.vmax = s->p_max;
+  r.vmin = s->p_min;
+  r.rms_sum = s->p_rms;
+  r.max_all = s->p_max_all;
+  r.min_all = s->p_min_all;
+  r.final_field = final_step ? s->p_final : nullptr;
+  return r;
+}
+
+Sens make_sens_u(const kw_sim* s, long long t, int a) {
+  Sens r{};
+  const bool rec = t >= s->start;
+  r.record = rec;
+  r.first = rec && (t == s->start);
+  r.t_rec = rec ? t - s->start : 0;
+  r.ns = s->ns;
+  r.row_ptr = s->s_row_ptr;
+  r.lin = s->s_lin;
+  r.raw = s->u_raw[a];
+  r.vmax = s->u_max[a];
+  r.vmin = s->u_min[a];
+  r.rms_sum = s->u_rms[a];
+  r.max_all = s->u_max_all[a];
+  r.min_all = s->u_min_all[a];
+  r.final_field = nullptr;
+  return r;
+}
+
+// One time step t (1-based) of kspaceFirstOrder3D.m:767-1206.
+void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
+  const Grid& g = s->g;
+  const int dim = g.dim;
+  const float invN = (float)(1.0 / (double)g.N);
+  Sens ps = make_sens_p(s, t, final_step);
+  Src nosrc[3] = {};
+  Sens us[3] = {};
+  for (int a = 0; a < dim; ++a) us[a] = make_sens_u(s, t, a);
+
+  if (t == 1 && s->p0) {
+    // p0 start (3D.m:1021-1035): p = p0, rho_i = p0/(dim c0^2),
+    // u = dt/2 rho_sg^-1 real(ifft(ddx_k_shift_pos kappa fft(p0)))
+    {
+      Prof pr(s, st, KK_XINIT);
+      launch_x_init(st, g, s->axX, s->twX, s->m, s->p0, s->rho, s->Pk, ps, s->xl);
+    }
+    float2* pk1[1] = {s->Pk};
+    ypass(s, st, pk1, 1, -1.f);
+    LastArgs a{};
+    a.in[0] = s->Pk;
+    a.nin = 1;
+    a.nout = dim;
+    for (int c = 0; c < dim; ++c) {
+      a.out[c] = s->G[c];
+      a.outs[c] = SpecOut{0, OP_DPOS_K, c};
+    }
+    a.scale = invN;
+    last(s, st, a);
+    ypass(s, st, s->G, dim, 1.f);
+    {
+      Prof pr(s, st, KK_XVEL);
+      launch_x_velocity(st, g, s->axX, s->twX, s->m, s->G, s->u, nosrc, us, 1, s->xl);
+    }
+    // restore the x spectrum of p0 for the next step
+    Sens none{};
+    none.row_ptr = s->s_row_ptr;
+    {
+      Prof pr(s, st, KK_XINIT);
+      launch_x_init(st, g, s->axX, s->twX, s->m, s->p0, s->rho, s->Pk, none, s->xl);
+    }
+    return;
+  }
+
+  // ---- velocity: u = pml_sg (pml_sg u - dt rho_sg^-1 real(ifft(D+ kappa p_k))) (3D.m:790-801)
+  float2* pk1[1] = {s->Pk};
+  ypass(s, st, pk1, 1, -1.f);
+  {
+    LastArgs a{};
+    a.in[0] = s->Pk;
+    a.nin = 1;
+    a.nout = dim;
+    for (int c = 0; c < dim; ++c) {
+      a.out[c] = s->G[c];
+      a.outs[c] = SpecOut{0, OP_DPOS_K, c};
+    }
+    a.scale = invN;
+    last(s, st, a);
+  }
+  ypass(s, st, s->G, dim, 1.f);
+  Src us_src[3] = {};
+  for (int c = 0; c < dim; ++c) {
+    us_src[c] = make_src(s, s->usrc[c], t);
+    if (us_src[c].active && us_src[c].mode == 2 && us_src[c].many) source_spectrum(s, st, us_src[c], 0, s->usrc[c].spec);
+  }
+  {
+    Prof pr(s, st, KK_XVEL);
+    launch_x_velocity(st, g, s->axX, s->twX, s->m, s->G, s->u, us_src, us, 0, s->xl);
+  }
+
+  // ---- divergence: du_i = real(ifft(D-_i kappa fft(u_i))) (3D.m:905-907)
+  ypass(s, st, s->G, dim, -1.f);
+  {
+    LastArgs a{};
+    a.nin = dim;
+    a.nout = dim;
+    for (int c = 0; c < dim; ++c) {
+      a.in[c] = s->G[c];
+      a.out[c] = s->G[c];
+      a.outs[c] = SpecOut{c, OP_DNEG_K, c};
+    }
+    a.scale = invN;
+    last(s, st, a);
+  }
+  ypass(s, st, s->G, dim, 1.f);
+
+  // ---- density, pressure source, equation of state (3D.m:910-1017)
+  Src psrc = make_src(s, s->psrc, t);
+  if (psrc.active && psrc.mode == 2 && psrc.many) source_spectrum(s, st, psrc, 0, s->psrc.spec);
+  {
+    Prof pr(s, st, KK_XDENS);
+    launch_x_density(st, g, s->axX, s->twX, s->m, s->G, s->rho, psrc, s->srho, s->Pk, ps, s->xl);
+  }
+  if (s->m.absorbing == 1) {
+    ypass(s, st, s->G, 2, -1.f);
+    LastArgs a{};
+    a.nin = 2;
+    a.nout = 2;
+    a.in[0] = a.out[0] = s->G[0];
+    a.in[1] = a.out[1] = s->G[1];
+    a.outs[0] = SpecOut{0, OP_NABLA1, 0};
+    a.outs[1] = SpecOut{1, OP_NABLA2, 0};
+    a.scale = invN;
+    last(s, st, a);
+    ypass(s, st, s->G, 2, 1.f);
+    Prof pr(s, st, KK_XPRES);
+    launch_x_pressure(st, g, s->axX, s->twX, s->m, s->G[0], s->G[1], s->srho, s->Pk, ps, s->xl);
+  }
+}
+
+kw_status setup_source(kw_sim* s, SrcData& d, uint64_t flag, uint64_t mode, uint64_t many, uint64_t npoints,
+                       const uint64_t* index, const float* input, const char* name,
+                       const std::vector<long long>* shared_lin, const std::vector<int>* shared_rows,
+                       int* shared_row_dev, long long* shared_lin_dev) {
+  d.flag = flag;
+  if (!flag) return KW_OK;
+  if (mode > 2) return fail(KW_ERR_INVALID, "%s mode %llu not in 0..2", name, (unsigned long long)mode);
+  if (!input) return fail(KW_ERR_INVALID, "%s input missing", name);
+  d.mode = (int)mode;
+  d.many = many ? 1 : 0;
+  d.npoints = npoints;
+  d.nseries = d.many ? npoints : 1;
+  (void)index;
+  d.row_ptr = shared_row_dev;
+  d.lin = shared_lin_dev;
+  (void)shared_lin;
+  (void)shared_rows;
+  d.sig = s->upload(input, (size_t)(flag * d.nseries));
+  if (!d.many) d.host_sig.assign(input, input + flag);
+  if (d.mode == 2) {
+    if (d.many) {
+      d.spec = s->alloc<float2>((size_t)s->nspec());
+    } else {
+      // corrected unit pattern real(ifft(source_kappa fft(mask))) (3D.m:815-822)
+      float2* tmp = s->alloc<float2>((size_t)s->nspec());
+      Src u{};
+      u.row_ptr = d.row_ptr;
+      u.lin = d.lin;
+      source_spectrum(s, s->stream, u, 1, tmp);
+      d.pattern = s->alloc<float>((size_t)s->g.N);
+      launch_x_c2r(s->stream, s->g, s->axX, s->twX, tmp, d.pattern, s->xl);
+    }
+  }
+  return KW_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+const char* kw_last_error(void) { return g_err.c_str(); }
+
+uint64_t kw_sim_footprint_bytes(const kw_sim_desc* d) {
+  if (!d || d->Nx == 0 || d->Ny == 0 || d->Nz == 0) return 0;
+  const uint64_t N = d->Nx * d->Ny * d->Nz;
+  const uint64_t dim = d->Nz > 1 ? 3 : 2;
+  const uint64_t nspec = (d->Nx / 2 + 1) * d->Ny * d->Nz;
+  uint64_t b = 0;
+  b += (2 * dim + 1) * N * 4;                  // u, rho, srho
+  b += (1 + dim) * nspec * 8;                  // Pk, G
+  b += 4 * nspec * 4;                          // spectral tables
+  auto fld = [&](const kw_real_array& a) { return a.count > 1 ? N * 4 : 0; };
+  b += fld(d->c0) + fld(d->rho0) + fld(d->rho0_sgx) + fld(d->rho0_sgy) + (dim == 3 ? fld(d->rho0_sgz) : 0);
+  b += fld(d->BonA) + 2 * fld(d->alpha_coeff) + (d->p0_source_input ? N * 4 : 0);
+  const uint64_t start = d->record_start_index ? d->record_start_index : 1;
+  const uint64_t nrec = d->Nt >= start ? d->Nt - start + 1 : 0;
+  const uint64_t ns = d->sensor_npoints;
+  if (d->record_flags & KW_REC_P) b += ns * nrec * 4;
+  if (d->record_flags & KW_REC_U) b += dim * ns * nrec * 4;
+  if (d->record_flags & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL | KW_REC_P_FINAL)) b += 3 * N * 4;
+  if (d->record_flags & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL)) b += 2 * dim * N * 4;
+  b += 8 * ns * 16;
+  return b;
+}
+
+kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
+  if (!d || !out) return fail(KW_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (d->abi_version != KW_ABI_VERSION)
+    return fail(KW_ERR_INVALID, "abi_version %u, library is %d", d->abi_version, KW_ABI_VERSION);
+  if (d->Nx < 1 || d->Ny < 1 || d->Nz < 1 || d->Nt < 1)
+    return fail(KW_ERR_INVALID, "grid %llux%llux%llu, Nt %llu", (unsigned long long)d->Nx,
+                (unsigned long long)d->Ny, (unsigned long long)d->Nz, (unsigned long long)d->Nt);
+  if (d->Nx < 2 || d->Ny < 2) return fail(KW_ERR_UNSUPPORTED, "1-D grids are not supported");
+  if (d->Nx > 1024) return fail(KW_ERR_UNSUPPORTED, "Nx = %llu exceeds 1024", (unsigned long long)d->Nx);
+  if (d->Ny > 8192 || d->Nz > 8192) return fail(KW_ERR_UNSUPPORTED, "Ny/Nz exceed 8192");
+  const long long N = (long long)(d->Nx * d->Ny * d->Nz);
+  if ((long long)(d->Ny * d->Nz) >= (1LL << 31)) return fail(KW_ERR_UNSUPPORTED, "too many rows");
+  if (!(d->dx > 0 && d->dy > 0 && d->dt > 0 && d->c_ref > 0))
+    return fail(KW_ERR_INVALID, "dx, dy, dt and c_ref must be positive");
+  if (d->Nz > 1 && !(d->dz > 0)) return fail(KW_ERR_INVALID, "dz must be positive");
+
+  kw_status st;
+  if ((st = check_array(d->c0, N, "c0", true))) return st;
+  if ((st = check_array(d->rho0, N, "rho0", true))) return st;
+  if ((st = check_array(d->rho0_sgx, N, "rho0_sgx", true))) return st;
+  if ((st = check_array(d->rho0_sgy, N, "rho0_sgy", true))) return st;
+  if (d->Nz > 1 && (st = check_array(d->rho0_sgz, N, "rho0_sgz", true))) return st;
+  if (d->nonlinear_flag && (st = check_array(d->BonA, N, "BonA", true))) return st;
+  if (d->absorbing_flag > 2) return fail(KW_ERR_INVALID, "absorbing_flag %llu", (unsigned long long)d->absorbing_flag);
+  if (d->absorbing_flag && (st = check_array(d->alpha_coeff, N, "alpha_coeff", true))) return st;
+  if (d->absorbing_flag == 1 && std::fabs(d->alpha_power - 1.0) < 1e-12)
+    return fail(KW_ERR_INVALID, "alpha_power = 1 is not supported by the power-law absorption model");
+
+  kw_sim* s = new kw_sim();
+  try {
+    Grid& g = s->g;
+    g.Nx = (int)d->Nx;
+    g.Ny = (int)d->Ny;
+    g.Nz = (int)d->Nz;
+    g.Nxh = g.Nx / 2 + 1;
+    g.dim = d->Nz > 1 ? 3 : 2;
+    g.rows = g.Ny * g.Nz;
+    g.N = N;
+    s->Nt = (long long)d->Nt;
+    s->start = d->record_start_index ? (long long)d->record_start_index : 1;
+    if (s->start > s->Nt) {
+      delete s;
+      return fail(KW_ERR_INVALID, "record_start_index %lld beyond Nt %lld", s->start, s->Nt);
+    }
+    s->rec = d->record_flags;
+    s->device = d->device;
+    if (!make_axis(g.Nx, &s->axX) || !make_axis(g.Ny, &s->axY) ||
+        (g.dim == 3 && !make_axis(g.Nz, &s->axL))) {
+      delete s;
+      return fail(KW_ERR_UNSUPPORTED, "grid %dx%dx%d: every dimension must factor into 2, 3, 5 and 7", g.Nx, g.Ny,
+                  g.Nz);
+    }
+    if (g.dim == 2) s->axL = s->axY;
+    s->xl = choose_x(g);
+    if (g.dim == 3) s->tly = choose_tile(g.Nxh, g.Ny, 2);
+    s->tll = choose_tile(g.dim == 3 ? g.Nxh * g.Ny : g.Nxh, s->axL.n, 3);
+    if (x_smem_bytes(g, s->xl.pairs, 3, true) > 65536) {
+      delete s;
+      return fail(KW_ERR_UNSUPPORTED, "x-row tile does not fit in LDS");
+    }
+
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&s->ev0));
+    HIPCHK(hipEventCreate(&s->ev1));
+    hipStream_t st0 = s->stream;
+
+    // twiddle tables (FP64 -> FP32)
+    s->twX = s->alloc<float2>(g.Nx);
+    launch_twiddles(st0, s->twX, g.Nx);
+    s->twY = s->alloc<float2>(g.Ny);
+    launch_twiddles(st0, s->twY, g.Ny);
+    if (g.dim == 3) {
+      s->twL = s->alloc<float2>(g.Nz);
+      launch_twiddles(st0, s->twL, g.Nz);
+    } else {
+      s->twL = s->twY;
+    }
+
+    // derivative / shift vectors (3D.m:627-634), spectral tables (3D.m:651-655)
+    const double dxs[3] = {d->dx, d->dy, g.dim == 3 ? d->dz : 1.0};
+    const int nfull[3] = {g.Nx, g.Ny, g.Nz};
+    const int nout[3] = {g.Nxh, g.Ny, g.Nz};
+    for (int a = 0; a < g.dim; ++a) {
+      float2* dp = s->alloc<float2>(nout[a]);
+      float2* dn = s->alloc<float2>(nout[a]);
+      launch_dvec(st0, dp, dn, nfull[a], nout[a], dxs[a]);
+      s->T.dpos[a] = dp;
+      s->T.dneg[a] = dn;
+    }
+    OpParams op{};
+    op.dx[0] = dxs[0];
+    op.dx[1] = dxs[1];
+    op.dx[2] = dxs[2];
+    op.dt = d->dt;
+    op.c_ref = d->c_ref;
+    op.alpha_power = d->alpha_power;
+    op.absorbing = (int)d->absorbing_flag;
+    float* kappa = s->alloc<float>((size_t)s->nspec());
+    const bool additive = (d->p_source_flag && d->p_source_mode == 2) ||
+                          ((d->ux_source_flag || d->uy_source_flag || d->uz_source_flag) && d->u_source_mode == 2);
+    float* skappa = additive ? s->alloc<float>((size_t)s->nspec()) : nullptr;
+    float* nab1 = d->absorbing_flag == 1 ? s->alloc<float>((size_t)s->nspec()) : nullptr;
+    float* nab2 = d->absorbing_flag == 1 ? s->alloc<float>((size_t)s->nspec()) : nullptr;
+    launch_spectral_tables(st0, g, op, kappa, skappa, nab1, nab2);
+    s->T.kappa = kappa;
+    s->T.skappa = skappa;
+    s->T.nabla1 = nab1;
+    s->T.nabla2 = nab2;
+
+    // PML (3D.m:617-622)
+    const uint64_t psz[3] = {d->pml_x_size, d->pml_y_size, d->pml_z_size};
+    const double pal[3] = {d->pml_x_alpha, d->pml_y_alpha, d->pml_z_alpha};
+    for (int a = 0; a < g.dim; ++a) {
+      if (2 * psz[a] > (uint64_t)nfull[a]) {
+        free_all(s);
+        delete s;
+        return fail(KW_ERR_INVALID, "PML size %llu too large for axis of %d points", (unsigned long long)psz[a],
+                    nfull[a]);
+      }
+      float* pm = s->alloc<float>(nfull[a]);
+      float* pms = s->alloc<float>(nfull[a]);
+      launch_pml(st0, pm, nfull[a], dxs[a], d->dt, d->c_ref, (int)psz[a], pal[a], 0);
+      launch_pml(st0, pms, nfull[a], dxs[a], d->dt, d->c_ref, (int)psz[a], pal[a], 1);
+      s->m.pml[a] = pm;
+      s->m.pml_sg[a] = pms;
+    }
+
+    // medium
+    Medium& m = s->m;
+    m.dt = (float)d->dt;
+    m.nonlinear = d->nonlinear_flag ? 1 : 0;
+    m.absorbing = (int)d->absorbing_flag;
+    m.c0 = make_sf(s, d->c0);
+    m.rho0 = make_sf(s, d->rho0);
+    const kw_real_array* sg[3] = {&d->rho0_sgx, &d->rho0_sgy, &d->rho0_sgz};
+    for (int a = 0; a < g.dim; ++a) {
+      if (sg[a]->count == 1) {
+        m.rsg[a].s = (float)(1.0 / (double)sg[a]->data[0]);
+      } else {
+        float* tmp = s->upload(sg[a]->data, (size_t)N);
+        float* inv = s->alloc<float>((size_t)N);
+        launch_reciprocal(st0, N, tmp, inv);
+        m.rsg[a].f = inv;
+      }
+    }
+    if (m.nonlinear) m.bona = make_sf(s, d->BonA);
+    if (m.absorbing) {
+      const bool field = d->c0.count > 1 || d->alpha_coeff.count > 1;
+      const float* c0d = m.c0.f;
+      const float* ad = d->alpha_coeff.count > 1 ? s->upload(d->alpha_coeff.data, (size_t)N) : nullptr;
+      const long long n = field ? N : 1;
+      float* tau = s->alloc<float>((size_t)n);
+      float* eta = s->alloc<float>((size_t)n);
+      launch_absorb_coeffs(st0, n, field ? c0d : nullptr, d->c0.data[0], ad, d->alpha_coeff.data[0],
+                           d->alpha_power, (int)d->absorbing_flag, tau, eta);
+      if (field) {
+        m.tau.f = tau;
+        m.eta.f = eta;
+      } else {
+        float h[2];
+        HIPCHK(hipMemcpyAsync(&h[0], tau, sizeof(float), hipMemcpyDeviceToHost, st0));
+        HIPCHK(hipMemcpyAsync(&h[1], eta, sizeof(float), hipMemcpyDeviceToHost, st0));
+        HIPCHK(hipStreamSynchronize(st0));
+        m.tau.s = h[0];
+        m.eta.s = m.absorbing == 1 ? h[1] : 0.f;
+      }
+    }
+
+    // state
+    for (int a = 0; a < g.dim; ++a) {
+      s->u[a] = s->alloc<float>((size_t)N);
+      s->rho[a] = s->alloc<float>((size_t)N);
+      s->G[a] = s->alloc<float2>((size_t)s->nspec());
+      s->zero(s->u[a], (size_t)N * 4);
+      s->zero(s->rho[a], (size_t)N * 4);
+    }
+    if (g.dim == 2) {  // the absorbing EOS uses G[0], G[1]; keep G[2] valid
+      s->G[2] = nullptr;
+    }
+    s->Pk = s->alloc<float2>((size_t)s->nspec());
+    s->zero(s->Pk, (size_t)s->nspec() * sizeof(float2));
+    if (m.absorbing == 1) s->srho = s->alloc<float>((size_t)N);
+    if (d->p0_source_input) s->p0 = s->upload(d->p0_source_input, (size_t)N);
+
+    // sensor
+    std::vector<long long> slin;
+    if ((st = load_indices(d->sensor_mask_index, d->sensor_npoints, N, "sensor_mask_index", slin))) {
+      free_all(s);
+      delete s;
+      return st;
+    }
+    s->ns = (int)d->sensor_npoints;
+    std::vector<int> srows;
+    build_rows(slin, g, srows);
+    s->s_row_ptr = s->upload(srows.data(), srows.size());
+    s->s_lin = s->upload(slin.data(), std::max<size_t>(slin.size(), 1));
+    s->nrec_total = s->Nt - s->start + 1;
+    const size_t nsz = std::max(s->ns, 1);
+    const uint64_t rf = s->rec;
+    if (rf & KW_REC_P) s->p_raw = s->alloc<float>(nsz * (size_t)s->nrec_total);
+    if (rf & KW_REC_P_MAX) s->p_max = s->alloc<float>(nsz);
+    if (rf & KW_REC_P_MIN) s->p_min = s->alloc<float>(nsz);
+    if (rf & KW_REC_P_RMS) s->p_rms = s->alloc<double>(2 * nsz);
+    if (rf & KW_REC_P_MAX_ALL) s->p_max_all = s->alloc<float>((size_t)N);
+    if (rf & KW_REC_P_MIN_ALL) s->p_min_all = s->alloc<float>((size_t)N);
+    if (rf & KW_REC_P_FINAL) s->p_final = s->alloc<float>((size_t)N);
+    for (int a = 0; a < g.dim; ++a) {
+      if (rf & KW_REC_U) s->u_raw[a] = s->alloc<float>(nsz * (size_t)s->nrec_total);
+      if (rf & KW_REC_U_MAX) s->u_max[a] = s->alloc<float>(nsz);
+      if (rf & KW_REC_U_MIN) s->u_min[a] = s->alloc<float>(nsz);
+      if (rf & KW_REC_U_RMS) s->u_rms[a] = s->alloc<double>(2 * nsz);
+      if (rf & KW_REC_U_MAX_ALL) s->u_max_all[a] = s->alloc<float>((size_t)N);
+      if (rf & KW_REC_U_MIN_ALL) s->u_min_all[a] = s->alloc<float>((size_t)N);
+    }
+
+    // sources (point lists share the row CSR construction)
+    if (d->p_source_flag) {
+      std::vector<long long> lin;
+      if ((st = load_indices(d->p_source_index, d->p_source_npoints, N, "p_source_index", lin))) {
+        free_all(s);
+        delete s;
+        return st;
+      }
+      std::vector<int> rows;
+      build_rows(lin, g, rows);
+      int* rd = s->upload(rows.data(), rows.size());
+      long long* ld = s->upload(lin.data(), lin.size());
+      if ((st = setup_source(s, s->psrc, d->p_source_flag, d->p_source_mode, d->p_source_many, d->p_source_npoints,
+                             d->p_source_index, d->p_source_input, "p_source", nullptr, nullptr, rd, ld))) {
+        free_all(s);
+        delete s;
+        return st;
+      }
+    }
+    const uint64_t uf[3] = {d->ux_source_flag, d->uy_source_flag, g.dim == 3 ? d->uz_source_flag : 0};
+    const float* uin[3] = {d->ux_source_input, d->uy_source_input, d->uz_source_input};
+    if (uf[0] || uf[1] || uf[2]) {
+      std::vector<long long> lin;
+      if ((st = load_indices(d->u_source_index, d->u_source_npoints, N, "u_source_index", lin))) {
+        free_all(s);
+        delete s;
+        return st;
+      }
+      std::vector<int> rows;
+      build_rows(lin, g, rows);
+      int* rd = s->upload(rows.data(), rows.size());
+      long long* ld = s->upload(lin.data(), lin.size());
+      float* shared_pattern = nullptr;
+      for (int a = 0; a < g.dim; ++a) {
+        if (!uf[a]) continue;
+        SrcData& sd = s->usrc[a];
+        if (shared_pattern && d->u_source_mode == 2 && !d->u_source_many) {
+          // the corrected pattern depends only on the (shared) positions
+          sd.flag = uf[a];
+          sd.mode = 2;
+          sd.many = 0;
+          sd.npoints = d->u_source_npoints;
+          sd.nseries = 1;
+          sd.row_ptr = rd;
+          sd.lin = ld;
+          sd.sig = s->upload(uin[a], (size_t)uf[a]);
+          sd.host_sig.assign(uin[a], uin[a] + uf[a]);
+          sd.pattern = shared_pattern;
+          continue;
+        }
+        if ((st = setup_source(s, sd, uf[a], d->u_source_mode, d->u_source_many, d->u_source_npoints,
+                               d->u_source_index, uin[a], "u_source", nullptr, nullptr, rd, ld))) {
+          free_all(s);
+          delete s;
+          return st;
+        }
+        shared_pattern = sd.pattern;
+      }
+    }
+    HIPCHK(hipStreamSynchronize(st0));
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    free_all(s);
+    delete s;
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  } catch (const OomError& e) {
+    free_all(s);
+    delete s;
+    return fail(KW_ERR_OOM, "device allocation of %zu bytes failed", e.bytes);
+  }
+  *out = s;
+  return KW_OK;
+}
+
+kw_status kw_sim_run(kw_sim* s, uint64_t nsteps, void* stream) {
+  if (!s) return fail(KW_ERR_INVALID, "null handle");
+  hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  try {
+    HIPCHK(hipSetDevice(s->device));
+    const long long end = std::min<long long>(s->Nt, s->t + (long long)nsteps);
+    for (long long t = s->t + 1; t <= end; ++t) step(s, st, t, t == end);
+    s->t = end;
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  }
+  return KW_OK;
+}
+
+kw_status kw_sim_sync(kw_sim* s) {
+  if (!s) return fail(KW_ERR_INVALID, "null handle");
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(KW_ERR_DEVICE, "HIP error %d (%s)", (int)e, hipGetErrorString(e));
+  return KW_OK;
+}
+
+uint64_t kw_sim_steps_done(const kw_sim* s) { return s ? (uint64_t)s->t : 0; }
+
+static const void* output_ptr(const kw_sim* s, int id, size_t* bytes, bool* is_rms) {
+  *is_rms = false;
+  const size_t N = (size_t)s->g.N;
+  const size_t ns = (size_t)s->ns;
+  const size_t nrec = (size_t)s->nrec_total;
+  if (id < 0 || id >= KW_OUT_COUNT) return nullptr;
+  if (id >= KW_OUT_UX) {
+    const int base = (id - KW_OUT_UX) / 3 * 3 + KW_OUT_UX;
+    const int a = id - base;
+    if (a >= s->g.dim) return nullptr;
+    switch (base) {
+      case KW_OUT_UX: *bytes = ns * nrec * 4; return s->u_raw[a];
+      case KW_OUT_UX_MAX: *bytes = ns * 4; return s->u_max[a];
+      case KW_OUT_UX_MIN: *bytes = ns * 4; return s->u_min[a];
+      case KW_OUT_UX_RMS: *bytes = ns * 4; *is_rms = true; return s->u_rms[a];
+      case KW_OUT_UX_MAX_ALL: *bytes = N * 4; return s->u_max_all[a];
+      case KW_OUT_UX_MIN_ALL: *bytes = N * 4; return s->u_min_all[a];
+      case KW_OUT_UX_FINAL: *bytes = N * 4; return (s->rec & KW_REC_U_FINAL) ? s->u[a] : nullptr;
+    }
+    return nullptr;
+  }
+  switch (id) {
+    case KW_OUT_P: *bytes = ns * nrec * 4; return s->p_raw;
+    case KW_OUT_P_MAX: *bytes = ns * 4; return s->p_max;
+    case KW_OUT_P_MIN: *bytes = ns * 4; return s->p_min;
+    case KW_OUT_P_RMS: *bytes = ns * 4; *is_rms = true; return s->p_rms;
+    case KW_OUT_P_MAX_ALL: *bytes = N * 4; return s->p_max_all;
+    case KW_OUT_P_MIN_ALL: *bytes = N * 4; return s->p_min_all;
+    case KW_OUT_P_FINAL: *bytes = N * 4; return s->p_final;
+  }
+  return nullptr;
+}
+
+size_t kw_sim_output_bytes(const kw_sim* s, int id) {
+  if (!s) return 0;
+  size_t b = 0;
+  bool rms;
+  return output_ptr(s, id, &b, &rms) ? b : 0;
+}
+
+kw_status kw_sim_fetch(kw_sim* s, int id, void* dst, size_t bytes) {
+  if (!s || !dst) return fail(KW_ERR_INVALID, "null argument");
+  size_t need = 0;
+  bool rms = false;
+  const void* src = output_ptr(s, id, &need, &rms);
+  if (!src) return fail(KW_ERR_INVALID, "output %d was not recorded", id);
+  if (bytes != need) return fail(KW_ERR_INVALID, "output %d needs %zu bytes, got %zu", id, need, bytes);
+  try {
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipDeviceSynchronize());
+    if (rms) {
+      const long long nrec_done = std::max<long long>(0, s->t - s->start + 1);
+      std::vector<double> h(2 * (size_t)s->ns);
+      HIPCHK(hipMemcpy(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+      float* o = static_cast<float*>(dst);
+      for (int j = 0; j < s->ns; ++j)
+        o[j] = nrec_done > 0 ? (float)std::sqrt(h[2 * (size_t)j] / (double)nrec_done) : 0.f;
+    } else {
+      HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    }
+  } catch (const HipError& e) {
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  }
+  return KW_OK;
+}
+
+kw_status kw_sim_destroy(kw_sim* s) {
+  if (!s) return KW_OK;
+  (void)hipSetDevice(s->device);
+  (void)hipDeviceSynchronize();
+  free_all(s);
+  delete s;
+  return KW_OK;
+}
+
+kw_status kw_sim_set_profiling(kw_sim* s, int enable) {
+  if (!s) return fail(KW_ERR_INVALID, "null handle");
+  s->profiling = enable != 0;
+  for (int k = 0; k < KK_COUNT; ++k) {
+    s->prof_ms[k] = 0.0;
+    s->prof_n[k] = 0;
+  }
+  return KW_OK;
+}
+
+int kw_sim_profile_get(kw_sim* s, const char** names, double* total_ms, uint64_t* launches, int cap) {
+  if (!s) return 0;
+  for (int k = 0; k < KK_COUNT && k < cap; ++k) {
+    if (names) names[k] = kKindNames[k];
+    if (total_ms) total_ms[k] = s->prof_ms[k];
+    if (launches) launches[k] = s->prof_n[k];
+  }
+  return KK_COUNT;
+}
+
+}  // extern "C"

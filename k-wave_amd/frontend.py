@@ -1,0 +1,251 @@
+"""Front-end mirror: builds the HDF5 input-file dataset set from k-Wave style
+(kgrid, medium, source, sensor) inputs, the way ``kspaceFirstOrder3D(...,
+'SaveToDisk', file)`` does before the external binary runs.
+
+Restated steps (only those that decide numbers in the file):
+
+* grid and wavenumbers          k-Wave/kWaveGrid.m:342-433, makeTime 612-671
+* reference sound speed         k-Wave/private/kspaceFirstOrder_setSoundSpeedRef.m:28-47
+* p0 smoothing                  k-Wave/private/kspaceFirstOrder_inputChecking.m:1499-1553
+* sensor / source indices       kspaceFirstOrder_inputChecking.m:1606-1659 (find(mask))
+* source scaling                k-Wave/private/kspaceFirstOrder_scaleSourceTerms.m:27-386
+* staggered density             k-Wave/kspaceFirstOrder3D.m:583-605, 2D.m:545-558
+* dataset names, flags, dtypes  k-Wave/private/kspaceFirstOrder_saveToDisk.m:44-414
+* PML defaults                  k-Wave/private/kspaceFirstOrder_setDefaults.m:149-168
+
+Grid arrays are given in MATLAB index order ((Nx, Ny) or (Nx, Ny, Nz)); the
+returned dict holds each dataset in HDF5 C order (MATLAB [a,b,c] -> (c,b,a)),
+float32 or uint64, exactly as ``writeMatrix`` stores it.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import geometry
+
+
+class KWaveGrid:
+    """Subset of kWaveGrid (kWaveGrid.m) needed to define a simulation."""
+
+    def __init__(self, Nx, dx, Ny, dy, Nz=None, dz=None):
+        self.Nx, self.dx, self.Ny, self.dy = int(Nx), float(dx), int(Ny), float(dy)
+        self.Nz = int(Nz) if Nz else 1
+        self.dz = float(dz) if dz else 0.0
+        self.dim = 3 if Nz else 2
+        self.Nt = None
+        self.dt = None
+
+    @property
+    def shape(self):
+        return (self.Nx, self.Ny) if self.dim == 2 else (self.Nx, self.Ny, self.Nz)
+
+    @property
+    def size(self):
+        return (self.Nx * self.dx, self.Ny * self.dy, self.Nz * self.dz)
+
+    def set_time(self, Nt, dt):
+        self.Nt, self.dt = int(Nt), float(dt)
+
+    def make_time(self, c, cfl=0.3, t_end=None):
+        """kWaveGrid.makeTime (kWaveGrid.m:612-671)."""
+        c = np.asarray(c, dtype=np.float64)
+        c_max, c_min = float(c.max()), float(c.min())
+        xs, ys, zs = self.size
+        if t_end is None:
+            t_end = math.sqrt(xs ** 2 + ys ** 2 + (zs ** 2 if self.dim == 3 else 0.0)) / c_min
+        dmin = min(self.dx, self.dy) if self.dim == 2 else min(self.dx, self.dy, self.dz)
+        self.dt = cfl * dmin / c_max
+        self.Nt = int(math.floor(t_end / self.dt)) + 1
+        if math.floor(t_end / self.dt) != math.ceil(t_end / self.dt) and math.fmod(t_end, self.dt) == 0:
+            self.Nt += 1
+        return self.dt, self.Nt
+
+
+def _c_order(a: np.ndarray) -> np.ndarray:
+    """MATLAB-order grid array -> HDF5 C-order (Nz, Ny, Nx) array."""
+    a = np.asarray(a)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    return np.ascontiguousarray(a.transpose(2, 1, 0))
+
+
+def _scalar(v, dtype=np.float32):
+    return np.full((1, 1, 1), v, dtype=dtype)
+
+
+def _find(mask: np.ndarray) -> np.ndarray:
+    """1-based column-major indices of nonzero entries (MATLAB find)."""
+    return np.flatnonzero(np.asarray(mask).reshape(-1, order="F")) + 1
+
+
+def _col(idx: np.ndarray) -> np.ndarray:
+    """MATLAB column vector [n,1] -> HDF5 (1,1,n) uint64."""
+    return np.asarray(idx, dtype=np.uint64).reshape(1, 1, -1)
+
+
+def _signal(sig: np.ndarray) -> np.ndarray:
+    """MATLAB [N_series, Nt_src] -> HDF5 (1, Nt_src, N_series) float32."""
+    s = np.atleast_2d(np.asarray(sig, dtype=np.float64))
+    return np.ascontiguousarray(s.T[None].astype(np.float32))
+
+
+def _stagger(rho0: np.ndarray, axis: int) -> np.ndarray:
+    """interpn('*linear') at +d/2 along axis, last plane copied (3D.m:583-596)."""
+    r = np.asarray(rho0, dtype=np.float64)
+    out = r.copy()
+    sl_a = [slice(None)] * r.ndim
+    sl_b = [slice(None)] * r.ndim
+    sl_a[axis] = slice(0, -1)
+    sl_b[axis] = slice(1, None)
+    out[tuple(sl_a)] = 0.5 * (r[tuple(sl_a)] + r[tuple(sl_b)])
+    return out
+
+
+def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict,
+                      pml_size=None, pml_alpha=2.0, smooth_p0: bool = True,
+                      sound_speed_ref=None) -> dict:
+    """Dataset dict of the input file (kspaceFirstOrder_saveToDisk.m:44-414).
+
+    medium: sound_speed, density, [alpha_coeff, alpha_power, alpha_mode='stokes'], [BonA]
+    source: [p0] | [p, p_mask, p_mode] | [ux/uy/uz, u_mask, u_mode]
+    sensor: mask (binary, MATLAB order)
+    """
+    dim = kgrid.dim
+    shape = kgrid.shape
+    if kgrid.Nt is None:
+        raise ValueError("kgrid time array not set")
+    if pml_size is None:
+        pml_size = 20 if dim == 2 else 10                       # setDefaults.m:149-168
+    ps = list(pml_size) if np.ndim(pml_size) else [int(pml_size)] * dim
+    pa = list(pml_alpha) if np.ndim(pml_alpha) else [float(pml_alpha)] * dim
+    for a in range(dim):
+        if 2 * ps[a] >= shape[a]:
+            raise ValueError("The size of the PML must be smaller than the size of the grid.")
+
+    c0 = np.asarray(medium["sound_speed"], dtype=np.float64)
+    rho0 = np.asarray(medium["density"], dtype=np.float64)
+    hetero_c = c0.size > 1
+    hetero_rho = rho0.size > 1
+    c0_s = float(c0.reshape(-1)[0]) if not hetero_c else None
+    if sound_speed_ref is None:
+        c_ref = float(c0.max())                                    # setSoundSpeedRef.m:44
+    elif sound_speed_ref == "min":
+        c_ref = float(c0.min())
+    elif sound_speed_ref == "mean":
+        c_ref = float(c0.mean())
+    elif sound_speed_ref == "max":
+        c_ref = float(c0.max())
+    else:
+        c_ref = float(sound_speed_ref)
+    dt = kgrid.dt
+    d = [kgrid.dx, kgrid.dy, kgrid.dz]
+    c_at = (lambda i: float(c0.reshape(-1, order="F")[i - 1])) if hetero_c else (lambda i: c0_s)
+
+    f = {}
+    f["Nx"], f["Ny"], f["Nz"], f["Nt"] = (_scalar(kgrid.Nx, np.uint64), _scalar(kgrid.Ny, np.uint64),
+                                          _scalar(kgrid.Nz if dim == 3 else 1, np.uint64),
+                                          _scalar(kgrid.Nt, np.uint64))
+    f["dx"], f["dy"] = _scalar(kgrid.dx), _scalar(kgrid.dy)
+    f["pml_x_size"], f["pml_y_size"] = _scalar(ps[0], np.uint64), _scalar(ps[1], np.uint64)
+    f["pml_z_size"] = _scalar(ps[2] if dim == 3 else 0, np.uint64)
+    f["pml_x_alpha"], f["pml_y_alpha"] = _scalar(pa[0]), _scalar(pa[1])
+    if dim == 3:
+        f["dz"] = _scalar(kgrid.dz)
+        f["pml_z_alpha"] = _scalar(pa[2])
+    f["dt"] = _scalar(dt)
+    f["c0"] = _c_order(c0).astype(np.float32) if hetero_c else _scalar(c0_s)
+    f["c_ref"] = _scalar(c_ref)
+    f["rho0"] = _c_order(rho0).astype(np.float32) if hetero_rho else _scalar(float(rho0.reshape(-1)[0]))
+    for a, nm in enumerate("xyz"[:dim]):
+        if hetero_rho:
+            f["rho0_sg" + nm] = _c_order(_stagger(rho0, a)).astype(np.float32)
+        else:
+            f["rho0_sg" + nm] = _scalar(float(rho0.reshape(-1)[0]))
+
+    # flags (writeFlags.m, saveToDisk.m:88-106)
+    zero = _scalar(0, np.uint64)
+    for nm in ("p_source_flag", "p0_source_flag", "ux_source_flag", "uy_source_flag",
+               "uz_source_flag", "sxx_source_flag", "syy_source_flag", "szz_source_flag",
+               "sxy_source_flag", "sxz_source_flag", "syz_source_flag", "transducer_source_flag",
+               "nonuniform_grid_flag", "nonlinear_flag", "absorbing_flag", "elastic_flag",
+               "axisymmetric_flag", "sensor_mask_type"):
+        f[nm] = zero.copy()
+
+    # nonlinearity
+    if "BonA" in medium and medium["BonA"] is not None:
+        f["nonlinear_flag"] = _scalar(1, np.uint64)
+        b = np.asarray(medium["BonA"], dtype=np.float64)
+        f["BonA"] = _c_order(b).astype(np.float32) if b.size > 1 else _scalar(float(b.reshape(-1)[0]))
+    # absorption (alpha_coeff stays in dB/(MHz^y cm): inputChecking.m:1666-1671)
+    if "alpha_coeff" in medium and medium["alpha_coeff"] is not None:
+        stokes = medium.get("alpha_mode") == "stokes"
+        f["absorbing_flag"] = _scalar(2 if stokes else 1, np.uint64)
+        ac = np.asarray(medium["alpha_coeff"], dtype=np.float64)
+        f["alpha_coeff"] = _c_order(ac).astype(np.float32) if ac.size > 1 else _scalar(float(ac.reshape(-1)[0]))
+        f["alpha_power"] = _scalar(float(medium.get("alpha_power", 2.0 if stokes else 1.5)))
+
+    # initial pressure (smoothed before expansion: inputChecking.m:1499-1553)
+    if source.get("p0") is not None:
+        p0 = np.asarray(source["p0"], dtype=np.float64)
+        if smooth_p0:
+            p0 = geometry.smooth(p0, True)
+        f["p0_source_flag"] = _scalar(1, np.uint64)
+        f["p0_source_input"] = _c_order(p0).astype(np.float32)
+
+    mode_code = {"dirichlet": 0, "additive-no-correction": 1, "additive": 2}
+    # pressure source (scaleSourceTerms.m:50-127)
+    if source.get("p") is not None:
+        sig = np.atleast_2d(np.asarray(source["p"], dtype=np.float64)).copy()
+        idx = _find(source["p_mask"])
+        mode = source.get("p_mode", "additive")
+        for r in range(sig.shape[0]):
+            c_here = c_at(idx[r]) if hetero_c else c0_s
+            if mode == "dirichlet":
+                sig[r] = sig[r] / (dim * c_here ** 2)
+            else:
+                sig[r] = sig[r] * (2 * dt / (dim * c_here * kgrid.dx))
+        f["p_source_flag"] = _scalar(sig.shape[1], np.uint64)
+        f["p_source_mode"] = _scalar(mode_code[mode], np.uint64)
+        f["p_source_many"] = _scalar(int(sig.shape[0] > 1), np.uint64)
+        f["p_source_index"] = _col(idx)
+        f["p_source_input"] = _signal(sig)
+
+    # velocity sources (scaleSourceTerms.m:281-353)
+    u_keys = [k for k in ("ux", "uy", "uz")[:dim] if source.get(k) is not None]
+    if u_keys:
+        idx = _find(source["u_mask"])
+        mode = source.get("u_mode", "additive")
+        many = None
+        for a, k in enumerate(("ux", "uy", "uz")[:dim]):
+            if source.get(k) is None:
+                continue
+            sig = np.atleast_2d(np.asarray(source[k], dtype=np.float64)).copy()
+            if mode != "dirichlet":
+                for r in range(sig.shape[0]):
+                    c_here = c_at(idx[r]) if hetero_c else c0_s
+                    sig[r] = sig[r] * (2 * c_here * dt / d[a])
+            if many is None:
+                many = int(sig.shape[0] > 1)
+            f[k + "_source_flag"] = _scalar(sig.shape[1], np.uint64)
+            f[k + "_source_input"] = _signal(sig)
+        f["u_source_mode"] = _scalar(mode_code[mode], np.uint64)
+        f["u_source_many"] = _scalar(many, np.uint64)
+        f["u_source_index"] = _col(idx)
+
+    # sensor (binary mask only: saveToDisk.m:35-37)
+    mask = np.asarray(sensor["mask"])
+    if mask.shape != shape:
+        raise ValueError("sensor.mask must be a binary grid mask of the grid's shape")
+    f["sensor_mask_index"] = _col(_find(mask))
+    return f
+
+
+def plane_wave_check_stable(kgrid: KWaveGrid, c_max: float) -> float:
+    """Largest stable dt for a lossless homogeneous medium (checkStability.m:78-143):
+    dt_limit = 2 / (c_ref * k_max) with k_max = pi * sqrt(sum 1/d^2)."""
+    inv = (1 / kgrid.dx ** 2 + 1 / kgrid.dy ** 2 + (1 / kgrid.dz ** 2 if kgrid.dim == 3 else 0))
+    kmax = math.pi * math.sqrt(inv)
+    return 2.0 / (c_max * kmax)

@@ -1,0 +1,318 @@
+"""ctypes binding of the solver C ABI (include/kwave_hip.h).
+
+``Simulation`` takes the dataset dict of an input file (the same dict
+``frontend.save_to_disk_dict`` builds and ``h5format`` reads/writes), creates
+a device-resident simulation, runs it and fetches outputs named like the
+datasets of the output file.  There is no fallback: if libkwave_hip.so is not
+built, or the HIP runtime reports no device, the calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libkwave_hip.so")
+
+KW_ABI_VERSION = 1
+
+RECORD_FLAGS = {
+    "p": 1 << 0, "p_max": 1 << 1, "p_min": 1 << 2, "p_rms": 1 << 3, "p_max_all": 1 << 4,
+    "p_min_all": 1 << 5, "p_final": 1 << 6, "u": 1 << 7, "u_max": 1 << 8, "u_min": 1 << 9,
+    "u_rms": 1 << 10, "u_max_all": 1 << 11, "u_min_all": 1 << 12, "u_final": 1 << 13,
+}
+
+# output ids (kw_output_id)
+_OUT_P = {"p": 0, "p_max": 1, "p_min": 2, "p_rms": 3, "p_max_all": 4, "p_min_all": 5, "p_final": 6}
+_OUT_U_BASE = {"": 7, "_max": 10, "_min": 13, "_rms": 16, "_max_all": 19, "_min_all": 22, "_final": 25}
+
+EXPORTED = (
+    "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
+    "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
+    "kw_sim_footprint_bytes",
+)
+
+
+class KwRealArray(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("count", C.c_uint64)]
+
+
+class KwSimDesc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("device", C.c_int32),
+        ("Nx", C.c_uint64), ("Ny", C.c_uint64), ("Nz", C.c_uint64), ("Nt", C.c_uint64),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double), ("c_ref", C.c_double),
+        ("pml_x_size", C.c_uint64), ("pml_y_size", C.c_uint64), ("pml_z_size", C.c_uint64),
+        ("pml_x_alpha", C.c_double), ("pml_y_alpha", C.c_double), ("pml_z_alpha", C.c_double),
+        ("c0", KwRealArray), ("rho0", KwRealArray), ("rho0_sgx", KwRealArray), ("rho0_sgy", KwRealArray),
+        ("rho0_sgz", KwRealArray),
+        ("nonlinear_flag", C.c_uint64), ("BonA", KwRealArray),
+        ("absorbing_flag", C.c_uint64), ("alpha_coeff", KwRealArray), ("alpha_power", C.c_double),
+        ("p0_source_input", C.POINTER(C.c_float)),
+        ("p_source_flag", C.c_uint64), ("p_source_mode", C.c_uint64), ("p_source_many", C.c_uint64),
+        ("p_source_npoints", C.c_uint64), ("p_source_index", C.POINTER(C.c_uint64)),
+        ("p_source_input", C.POINTER(C.c_float)),
+        ("ux_source_flag", C.c_uint64), ("uy_source_flag", C.c_uint64), ("uz_source_flag", C.c_uint64),
+        ("u_source_mode", C.c_uint64), ("u_source_many", C.c_uint64), ("u_source_npoints", C.c_uint64),
+        ("u_source_index", C.POINTER(C.c_uint64)),
+        ("ux_source_input", C.POINTER(C.c_float)), ("uy_source_input", C.POINTER(C.c_float)),
+        ("uz_source_input", C.POINTER(C.c_float)),
+        ("sensor_npoints", C.c_uint64), ("sensor_mask_index", C.POINTER(C.c_uint64)),
+        ("record_flags", C.c_uint64), ("record_start_index", C.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libkwave_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"native solver library missing: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    lib.kw_sim_create.argtypes = [C.POINTER(KwSimDesc), C.POINTER(C.c_void_p)]
+    lib.kw_sim_create.restype = C.c_int
+    lib.kw_sim_run.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+    lib.kw_sim_run.restype = C.c_int
+    lib.kw_sim_sync.argtypes = [C.c_void_p]
+    lib.kw_sim_sync.restype = C.c_int
+    lib.kw_sim_steps_done.argtypes = [C.c_void_p]
+    lib.kw_sim_steps_done.restype = C.c_uint64
+    lib.kw_sim_output_bytes.argtypes = [C.c_void_p, C.c_int]
+    lib.kw_sim_output_bytes.restype = C.c_size_t
+    lib.kw_sim_fetch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+    lib.kw_sim_fetch.restype = C.c_int
+    lib.kw_sim_destroy.argtypes = [C.c_void_p]
+    lib.kw_sim_destroy.restype = C.c_int
+    lib.kw_last_error.argtypes = []
+    lib.kw_last_error.restype = C.c_char_p
+    lib.kw_sim_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    lib.kw_sim_set_profiling.restype = C.c_int
+    lib.kw_sim_profile_get.argtypes = [C.c_void_p, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_uint64), C.c_int]
+    lib.kw_sim_profile_get.restype = C.c_int
+    lib.kw_sim_footprint_bytes.argtypes = [C.POINTER(KwSimDesc)]
+    lib.kw_sim_footprint_bytes.restype = C.c_uint64
+    _lib = lib
+    return lib
+
+
+class KWaveError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"kw status {status}: {msg}")
+        self.status = status
+
+
+def _check(lib, st):
+    if st != 0:
+        raise KWaveError(st, lib.kw_last_error().decode(errors="replace"))
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+
+
+def _u64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64).reshape(-1))
+
+
+def _int(v, default=0):
+    if v is None:
+        return default
+    return int(np.asarray(v).reshape(-1)[0])
+
+
+def _flt(v, default=0.0):
+    if v is None:
+        return default
+    return float(np.asarray(v, dtype=np.float64).reshape(-1)[0])
+
+
+def build_desc(f: dict, record=("p",), record_start_index: int = 1, device: int = 0):
+    """kw_sim_desc from an input-file dataset dict; returns (desc, keepalive)."""
+    keep = []
+
+    def ptr_f(a):
+        if a is None:
+            return None
+        arr = _f32(a)
+        keep.append(arr)
+        return arr.ctypes.data_as(C.POINTER(C.c_float))
+
+    def ptr_u(a):
+        if a is None:
+            return None
+        arr = _u64(a)
+        keep.append(arr)
+        return arr.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def real(name):
+        if name not in f:
+            return KwRealArray(None, 0)
+        arr = _f32(f[name])
+        keep.append(arr)
+        return KwRealArray(arr.ctypes.data_as(C.POINTER(C.c_float)), arr.size)
+
+    d = KwSimDesc()
+    d.abi_version = KW_ABI_VERSION
+    d.device = device
+    d.Nx, d.Ny, d.Nz, d.Nt = _int(f["Nx"]), _int(f["Ny"]), _int(f.get("Nz"), 1), _int(f["Nt"])
+    d.dx, d.dy, d.dz = _flt(f["dx"]), _flt(f["dy"]), _flt(f.get("dz"), 0.0)
+    d.dt, d.c_ref = _flt(f["dt"]), _flt(f["c_ref"])
+    d.pml_x_size, d.pml_y_size, d.pml_z_size = (_int(f.get("pml_x_size")), _int(f.get("pml_y_size")),
+                                                _int(f.get("pml_z_size")))
+    d.pml_x_alpha, d.pml_y_alpha, d.pml_z_alpha = (_flt(f.get("pml_x_alpha")), _flt(f.get("pml_y_alpha")),
+                                                   _flt(f.get("pml_z_alpha")))
+    d.c0, d.rho0 = real("c0"), real("rho0")
+    d.rho0_sgx, d.rho0_sgy, d.rho0_sgz = real("rho0_sgx"), real("rho0_sgy"), real("rho0_sgz")
+    d.nonlinear_flag = _int(f.get("nonlinear_flag"))
+    d.BonA = real("BonA")
+    d.absorbing_flag = _int(f.get("absorbing_flag"))
+    d.alpha_coeff = real("alpha_coeff")
+    d.alpha_power = _flt(f.get("alpha_power"))
+    if _int(f.get("p0_source_flag")):
+        d.p0_source_input = ptr_f(f["p0_source_input"])
+    d.p_source_flag = _int(f.get("p_source_flag"))
+    if d.p_source_flag:
+        d.p_source_mode = _int(f.get("p_source_mode"))
+        d.p_source_many = _int(f.get("p_source_many"))
+        d.p_source_npoints = np.asarray(f["p_source_index"]).size
+        d.p_source_index = ptr_u(f["p_source_index"])
+        d.p_source_input = ptr_f(f["p_source_input"])
+    d.ux_source_flag = _int(f.get("ux_source_flag"))
+    d.uy_source_flag = _int(f.get("uy_source_flag"))
+    d.uz_source_flag = _int(f.get("uz_source_flag"))
+    if d.ux_source_flag or d.uy_source_flag or d.uz_source_flag:
+        d.u_source_mode = _int(f.get("u_source_mode"))
+        d.u_source_many = _int(f.get("u_source_many"))
+        d.u_source_npoints = np.asarray(f["u_source_index"]).size
+        d.u_source_index = ptr_u(f["u_source_index"])
+        for k in ("ux", "uy", "uz"):
+            if _int(f.get(k + "_source_flag")):
+                setattr(d, k + "_source_input", ptr_f(f[k + "_source_input"]))
+    sm = f["sensor_mask_index"]
+    d.sensor_npoints = np.asarray(sm).size
+    d.sensor_mask_index = ptr_u(sm)
+    flags = 0
+    for r in record:
+        if r not in RECORD_FLAGS:
+            raise ValueError(f"unknown record option {r}")
+        flags |= RECORD_FLAGS[r]
+    d.record_flags = flags
+    d.record_start_index = int(record_start_index)
+    return d, keep
+
+
+def output_names(record, dim: int):
+    names = []
+    for r in record:
+        if r.startswith("u"):
+            suffix = r[1:]
+            names += ["u" + a + suffix for a in "xyz"[:dim]]
+        else:
+            names.append(r)
+    return names
+
+
+def output_id(name: str) -> int:
+    if name in _OUT_P:
+        return _OUT_P[name]
+    if name[:2] in ("ux", "uy", "uz"):
+        axis = "xyz".index(name[1])
+        return _OUT_U_BASE[name[2:]] + axis
+    raise ValueError(f"unknown output {name}")
+
+
+class Simulation:
+    """One device-resident simulation (kw_sim handle)."""
+
+    def __init__(self, f: dict, record=("p",), record_start_index: int = 1, device: int = 0):
+        self.lib = load_library()
+        self.desc, keep = build_desc(f, record, record_start_index, device)
+        self.record = tuple(record)
+        self.Nx, self.Ny, self.Nz = int(self.desc.Nx), int(self.desc.Ny), int(self.desc.Nz)
+        self.dim = 3 if self.Nz > 1 else 2
+        self.Nt = int(self.desc.Nt)
+        self.ns = int(self.desc.sensor_npoints)
+        self.nrec = self.Nt - max(1, int(record_start_index)) + 1
+        h = C.c_void_p()
+        _check(self.lib, self.lib.kw_sim_create(C.byref(self.desc), C.byref(h)))
+        del keep
+        self.handle = h
+
+    def run(self, nsteps: int | None = None, stream: int | None = None):
+        n = self.Nt if nsteps is None else int(nsteps)
+        _check(self.lib, self.lib.kw_sim_run(self.handle, n, C.c_void_p(stream) if stream else None))
+        return self
+
+    def sync(self):
+        _check(self.lib, self.lib.kw_sim_sync(self.handle))
+
+    @property
+    def steps_done(self) -> int:
+        return int(self.lib.kw_sim_steps_done(self.handle))
+
+    def fetch(self, name: str) -> np.ndarray:
+        oid = output_id(name)
+        nbytes = self.lib.kw_sim_output_bytes(self.handle, oid)
+        if nbytes == 0:
+            raise KWaveError(1, f"output {name} was not recorded")
+        out = np.empty(nbytes // 4, dtype=np.float32)
+        _check(self.lib, self.lib.kw_sim_fetch(self.handle, oid, out.ctypes.data, nbytes))
+        if name in ("p",) or name in ("ux", "uy", "uz"):
+            return out.reshape(self.nrec, self.ns)
+        if name.endswith("_all") or name.endswith("_final"):
+            return out.reshape(self.Nz, self.Ny, self.Nx)
+        return out
+
+    def outputs(self) -> dict:
+        return {n: self.fetch(n) for n in output_names(self.record, self.dim)}
+
+    def set_profiling(self, on: bool = True):
+        _check(self.lib, self.lib.kw_sim_set_profiling(self.handle, 1 if on else 0))
+
+    def profile(self) -> dict:
+        cap = 16
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        n = (C.c_uint64 * cap)()
+        k = self.lib.kw_sim_profile_get(self.handle, names, ms, n, cap)
+        return {names[i].decode(): (ms[i], int(n[i])) for i in range(min(k, cap))}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.kw_sim_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run(f: dict, record=("p",), record_start_index: int = 1, device: int = 0) -> dict:
+    """Run a whole simulation and return its outputs (output-file naming)."""
+    with Simulation(f, record, record_start_index, device) as sim:
+        sim.run()
+        return sim.outputs()
+
+
+def footprint_bytes(f: dict, record=("p",), record_start_index: int = 1) -> int:
+    lib = load_library()
+    d, keep = build_desc(f, record, record_start_index)
+    n = int(lib.kw_sim_footprint_bytes(C.byref(d)))
+    del keep
+    return n

@@ -1,0 +1,110 @@
+"""Seeded simulation inputs shared by the oracle and the GPU parity tests.
+
+Every case is built through the front-end mirror (frontend.save_to_disk_dict)
+so the oracle and the HIP solver read the identical input-file dataset dict.
+Sizes are chosen so the FP64 oracle finishes in a few seconds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+import kwave_amd as kw
+
+fe = kw.frontend
+geo = kw.geometry
+
+
+def smooth_field(rng, shape, sigma=2.0):
+    """Unit-std Gaussian random field: white noise filtered by a Gaussian of sigma points."""
+    w = rng.standard_normal(shape)
+    k = [np.fft.fftfreq(n) for n in shape]
+    grids = np.meshgrid(*k, indexing="ij")
+    g = np.exp(-2 * (np.pi * sigma) ** 2 * sum(x ** 2 for x in grids))
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * g))
+    return f / f.std()
+
+
+def case_2d_p0_lossless(N=64, Nt=80, pml=10, het=False, seed=1):
+    rng = np.random.default_rng(seed)
+    kgrid = fe.KWaveGrid(N, 1e-4, N, 1e-4)
+    c = 1500.0 * (1 + (0.05 * smooth_field(rng, (N, N)) if het else 0.0))
+    rho = 1000.0 * (1 + (0.05 * smooth_field(rng, (N, N)) if het else 0.0))
+    kgrid.set_time(Nt, 0.3 * 1e-4 / float(np.max(c)))
+    p0 = 5.0 * geo.make_disc(N, N, 0, 0, 5)
+    mask = (rng.random((N, N)) < 0.02).astype(float)
+    return fe.save_to_disk_dict(kgrid, {"sound_speed": c, "density": rho}, {"p0": p0}, {"mask": mask},
+                                pml_size=pml)
+
+
+def case_2d_absorbing_psource(N=(60, 48), Nt=120, mode="additive", many=False, nonlinear=False,
+                              stokes=False, seed=2):
+    rng = np.random.default_rng(seed)
+    Nx, Ny = N
+    kgrid = fe.KWaveGrid(Nx, 1e-4, Ny, 1.2e-4)
+    c = 1500.0 * (1 + 0.04 * smooth_field(rng, (Nx, Ny)))
+    rho = 1000.0 * (1 + 0.04 * smooth_field(rng, (Nx, Ny)))
+    kgrid.set_time(Nt, 0.25 * 1e-4 / float(np.max(c)))
+    medium = {"sound_speed": c, "density": rho,
+              "alpha_coeff": 0.5 + 0.2 * rng.random((Nx, Ny)), "alpha_power": 2.0 if stokes else 1.5}
+    if stokes:
+        medium["alpha_mode"] = "stokes"
+    if nonlinear:
+        medium["BonA"] = 6.0
+    pmask = np.zeros((Nx, Ny))
+    pmask[15, 10:38] = 1
+    npts = int(pmask.sum())
+    burst = geo.tone_burst(1 / kgrid.dt, 1.5e6, 3)
+    if many:
+        sig = np.vstack([np.roll(burst[0], k) for k in range(npts)]) * 1e5
+    else:
+        sig = burst * 1e5
+    mask = np.zeros((Nx, Ny))
+    mask[40, :] = 1
+    mask[:, 30] = 1
+    return fe.save_to_disk_dict(kgrid, medium, {"p": sig, "p_mask": pmask, "p_mode": mode}, {"mask": mask},
+                                pml_size=8)
+
+
+def case_3d(N=(32, 28, 24), Nt=60, absorbing=True, nonlinear=False, source="ux", mode="additive", many=False,
+            het=True, p0=False, seed=3, alpha_power=1.5, pml=6):
+    rng = np.random.default_rng(seed)
+    Nx, Ny, Nz = N
+    dx = 2e-4
+    kgrid = fe.KWaveGrid(Nx, dx, Ny, dx, Nz, dx)
+    shape = (Nx, Ny, Nz)
+    c = 1500.0 * (1 + (0.03 * smooth_field(rng, shape) if het else 0.0))
+    rho = 1000.0 * (1 + (0.03 * smooth_field(rng, shape) if het else 0.0))
+    kgrid.set_time(Nt, 0.3 * dx / float(np.max(c)))
+    medium = {"sound_speed": c, "density": rho}
+    if absorbing:
+        medium["alpha_coeff"] = (0.5 + 0.25 * rng.random(shape)) if het else 0.75
+        medium["alpha_power"] = alpha_power
+    if nonlinear:
+        medium["BonA"] = 5.0
+    src = {}
+    if p0:
+        src["p0"] = 3.0 * geo.make_ball(Nx, Ny, Nz, 0, 0, 0, 3)
+    if source:
+        m = np.zeros(shape)
+        m[pml + 2, Ny // 2 - 3: Ny // 2 + 3, Nz // 2 - 3: Nz // 2 + 3] = 1
+        npts = int(m.sum())
+        burst = geo.tone_burst(1 / kgrid.dt, 1e6, 3)[0]
+        sig = np.vstack([np.roll(burst, k) for k in range(npts)]) if many else burst[None]
+        if source == "p":
+            src.update({"p": sig * 1e5, "p_mask": m, "p_mode": mode})
+        else:
+            src.update({"ux": sig * 0.5, "u_mask": m, "u_mode": mode})
+            if source == "uxyz":
+                src["uy"] = sig * 0.3
+                src["uz"] = sig * 0.2
+    mask = np.zeros(shape)
+    mask[:, :, Nz // 2] = 1
+    return fe.save_to_disk_dict(kgrid, medium, src, {"mask": mask}, pml_size=pml)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)

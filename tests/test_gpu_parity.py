@@ -1,0 +1,83 @@
+"""GPU parity: the HIP solver (through the C ABI) against the FP64 oracle.
+
+Tolerance: relative L2 <= 1e-4 on every recorded output (the FP32 drop-in
+bar), plus a max-abs bound relative to the output's peak.  Each case prints
+its name before running so a failing GPU step is identifiable.
+"""
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL_L2 = 1e-4
+
+
+def compare(kw, oracle, f, record, name, tol=TOL_L2, start=1):
+    print(f"[gpu parity] {name} record={record}", flush=True)
+    gpu = kw.run(f, record=record, record_start_index=start)
+    ref = oracle.run(f, record=record, record_start_index=start)
+    assert set(ref) <= set(gpu) | set(), (sorted(ref), sorted(gpu))
+    for k, r in ref.items():
+        g = gpu[k]
+        assert g.shape == r.shape, (k, g.shape, r.shape)
+        e = cases.rel_l2(g, r)
+        mx = np.max(np.abs(g - r)) / max(np.max(np.abs(r)), 1e-30)
+        print(f"    {k}: rel L2 {e:.2e}  max-abs/peak {mx:.2e}", flush=True)
+        assert e <= tol, (name, k, e)
+        assert mx <= 10 * tol, (name, k, mx)
+    return gpu, ref
+
+
+def test_2d_p0_lossless_homogeneous(kw, oracle):
+    f = cases.case_2d_p0_lossless(N=64, Nt=80)
+    compare(kw, oracle, f, ("p", "p_max", "p_min", "p_rms", "p_final"), "2d_p0_homog")
+
+
+def test_2d_p0_heterogeneous_velocity_outputs(kw, oracle):
+    f = cases.case_2d_p0_lossless(N=64, Nt=60, het=True)
+    compare(kw, oracle, f, ("p", "u", "u_max", "u_rms", "u_final", "p_max_all", "p_min_all"), "2d_p0_het")
+
+
+@pytest.mark.parametrize("mode", ["additive", "additive-no-correction", "dirichlet"])
+def test_2d_absorbing_pressure_source(kw, oracle, mode):
+    f = cases.case_2d_absorbing_psource(mode=mode)
+    compare(kw, oracle, f, ("p", "p_max"), f"2d_abs_p_{mode}")
+
+
+def test_2d_many_additive_nonlinear(kw, oracle):
+    f = cases.case_2d_absorbing_psource(many=True, nonlinear=True)
+    compare(kw, oracle, f, ("p", "p_rms"), "2d_many_nonlinear")
+
+
+def test_2d_stokes(kw, oracle):
+    f = cases.case_2d_absorbing_psource(stokes=True)
+    compare(kw, oracle, f, ("p",), "2d_stokes")
+
+
+def test_3d_absorbing_heterogeneous_ux_additive(kw, oracle):
+    f = cases.case_3d()
+    compare(kw, oracle, f, ("p", "p_max", "u"), "3d_abs_ux_add")
+
+
+@pytest.mark.parametrize("mode", ["dirichlet", "additive-no-correction"])
+def test_3d_velocity_modes(kw, oracle, mode):
+    f = cases.case_3d(mode=mode, source="uxyz", Nt=40)
+    compare(kw, oracle, f, ("p", "u_max"), f"3d_u_{mode}")
+
+
+def test_3d_nonlinear_p_source_many(kw, oracle):
+    f = cases.case_3d(N=(30, 28, 36), source="p", many=True, nonlinear=True, alpha_power=2.0, Nt=50)
+    compare(kw, oracle, f, ("p", "p_max", "p_rms"), "3d_nl_p_many")
+
+
+def test_3d_p0_lossless_odd_grid(kw, oracle):
+    f = cases.case_3d(N=(25, 21, 35), absorbing=False, source=None, p0=True, Nt=40, pml=4)
+    compare(kw, oracle, f, ("p", "p_final", "u_final"), "3d_p0_odd")
+
+
+def test_record_start_index(kw, oracle):
+    f = cases.case_2d_p0_lossless(N=48, Nt=50)
+    compare(kw, oracle, f, ("p", "p_max", "p_rms", "p_max_all"), "2d_start_index", start=20)
