@@ -1,0 +1,89 @@
+"""Seeded synthetic workloads (input-file dataset dicts).
+
+C1  2-D 128^2 homogeneous lossless p0, Cartesian circle sensor (p)
+C2  3-D 128^3 heterogeneous absorbing, 16x16 ux additive patch, mid-plane p + p_max
+C3  3-D 256x256x512 nonlinear absorbing (y=2), bowl p source, all-point p_max + p_rms
+C4  3-D 768^3 heterogeneous absorbing three-layer slab, 64x64 p patch, 128^2 plane p
+C5  serving stream: requests on 64^3 .. 160x160x192 grids, 1% random sensor
+
+RNG: numpy.random.default_rng(1000 + config number), draws in the order
+written below.  Random media are unit-std Gaussian random fields: white noise
+filtered by a Gaussian of sigma = correlation length (in points), rescaled.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import frontend as fe
+from . import geometry as geo
+
+
+def gaussian_random_field(rng, shape, corr_len: float) -> np.ndarray:
+    """White noise filtered by a Gaussian of std corr_len points, unit std."""
+    w = rng.standard_normal(shape)
+    ks = np.meshgrid(*[np.fft.fftfreq(n) for n in shape], indexing="ij", sparse=True)
+    g = np.exp(-2.0 * (math.pi * corr_len) ** 2 * sum(k ** 2 for k in ks))
+    f = np.fft.ifftn(np.fft.fftn(w) * g).real
+    return f / f.std()
+
+
+def c2(Nt: int | None = None, N: int = 128) -> dict:
+    """3-D heterogeneous absorbing linear medium (128^3, PML 10, dx = 0.2 mm)."""
+    rng = np.random.default_rng(1002)
+    dx = 0.2e-3
+    shape = (N, N, N)
+    kgrid = fe.KWaveGrid(N, dx, N, dx, N, dx)
+    g1 = gaussian_random_field(rng, shape, 4.0)
+    g2 = gaussian_random_field(rng, shape, 4.0)
+    c = 1500.0 * (1 + 0.02 * g1)
+    rho = 1000.0 * (1 + 0.02 * g2)
+    alpha = np.full(shape, 0.0022)
+    ball = geo.make_ball(N, N, N, 0, 0, 0, 20 * N // 128) > 0
+    c[ball], rho[ball], alpha[ball] = 1580.0, 1050.0, 0.75
+    dt = 0.3 * dx / float(c.max())
+    kgrid.set_time(1200 if Nt is None else Nt, dt)
+    u_mask = np.zeros(shape)
+    h = 8 * N // 128
+    u_mask[10, N // 2 - h: N // 2 + h, N // 2 - h: N // 2 + h] = 1       # x = 11 (1-based)
+    ux = geo.tone_burst(1.0 / dt, 1e6, 3)
+    sensor = np.zeros(shape)
+    sensor[:, :, N // 2 - 1] = 1                                           # z = 64 (1-based)
+    medium = {"sound_speed": c, "density": rho, "alpha_coeff": alpha, "alpha_power": 1.5}
+    return fe.save_to_disk_dict(kgrid, medium, {"ux": ux, "u_mask": u_mask, "u_mode": "additive"},
+                                {"mask": sensor}, pml_size=10)
+
+
+C2_RECORD = ("p", "p_max")
+C2_DESCRIPTION = ("3-D 128^3 heterogeneous absorbing (y=1.5) linear medium, PML 10, 16x16 ux additive "
+                  "tone-burst patch, mid-plane binary sensor recording p and p_max, CFL 0.3, Nt 1200")
+
+
+def c4(Nt: int | None = None, N: int = 768) -> dict:
+    """3-D three-layer slab, heterogeneous absorbing (y=1.1), PML 20, dx = 0.2 mm."""
+    rng = np.random.default_rng(1004)
+    dx = 0.2e-3
+    shape = (N, N, N)
+    kgrid = fe.KWaveGrid(N, dx, N, dx, N, dx)
+    pml = 20
+    interior = N - 2 * pml
+    x = np.arange(N)
+    layer = np.where(x < pml + interior // 3, 0, np.where(x < pml + 2 * interior // 3, 1, 2))
+    cs, rs, al = np.array([1480.0, 1560.0, 2800.0]), np.array([1000.0, 1050.0, 1900.0]), np.array([0.0022, 0.6, 4.0])
+    pert = 1 + 0.01 * gaussian_random_field(rng, shape, 4.0).astype(np.float32)
+    c = (cs[layer][:, None, None] * pert).astype(np.float32)
+    rho = (rs[layer][:, None, None] * pert).astype(np.float32)
+    del pert
+    alpha = np.broadcast_to(al[layer][:, None, None], shape).astype(np.float32)
+    dt = 0.3 * dx / float(c.max())
+    kgrid.set_time(1500 if Nt is None else Nt, dt)
+    p_mask = np.zeros(shape, dtype=np.float32)
+    p_mask[pml, N // 2 - 32: N // 2 + 32, N // 2 - 32: N // 2 + 32] = 1   # x = 21 (1-based)
+    sig = geo.tone_burst(1.0 / dt, 1e6, 4)
+    sensor = np.zeros(shape, dtype=np.float32)
+    sensor[N - pml - 1, N // 2 - 64: N // 2 + 64, N // 2 - 64: N // 2 + 64] = 1
+    medium = {"sound_speed": c, "density": rho, "alpha_coeff": alpha, "alpha_power": 1.1}
+    return fe.save_to_disk_dict(kgrid, medium, {"p": sig, "p_mask": p_mask, "p_mode": "additive"},
+                                {"mask": sensor}, pml_size=pml)

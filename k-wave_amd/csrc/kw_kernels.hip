@@ -827,6 +827,17 @@ void launch_reciprocal(hipStream_t s, long long n, const float* in, float* out) 
   hipLaunchKernelGGL(k_reciprocal, dim3(nblk(n, 256)), dim3(256), 0, s, n, in, out);
 }
 
+// plain vector copy (counter calibration: FETCH_SIZE/WRITE_SIZE of a known byte count)
+__global__ void k_copy(const float4* __restrict__ in, float4* __restrict__ out, long long n4) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+void launch_copy(hipStream_t s, const void* in, void* out, size_t bytes) {
+  const long long n4 = (long long)(bytes / 16);
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, (const float4*)in, (float4*)out, n4);
+}
+
 void launch_finish_rms(hipStream_t s, long long n, const double* sum, double count, float* out) {
   hipLaunchKernelGGL(k_finish_rms, dim3(nblk(n, 256)), dim3(256), 0, s, n, sum, count, out);
 }

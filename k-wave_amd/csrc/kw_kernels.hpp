@@ -135,5 +135,6 @@ void launch_absorb_coeffs(hipStream_t s, long long n, const float* c0, float c0s
                           float alphas, double y, int absorbing, float* tau, float* eta);
 void launch_reciprocal(hipStream_t s, long long n, const float* in, float* out);
 void launch_finish_rms(hipStream_t s, long long n, const double* sum, double count, float* out);
+void launch_copy(hipStream_t s, const void* in, void* out, size_t bytes);
 
 }  // namespace kw

@@ -932,4 +932,33 @@ int kw_sim_profile_get(kw_sim* s, const char** names, double* total_ms, uint64_t
   return KK_COUNT;
 }
 
+kw_status kw_diag_copy(uint64_t bytes, int device, double* ms) {
+  void *a = nullptr, *b = nullptr;
+  try {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMalloc(&a, bytes));
+    HIPCHK(hipMalloc(&b, bytes));
+    HIPCHK(hipMemset(a, 0, bytes));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, nullptr));
+    launch_copy(nullptr, a, b, (size_t)bytes);
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  } catch (const HipError& e) {
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return KW_OK;
+}
+
 }  // extern "C"

@@ -32,7 +32,7 @@ _OUT_U_BASE = {"": 7, "_max": 10, "_min": 13, "_rms": 16, "_max_all": 19, "_min_
 EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
     "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
-    "kw_sim_footprint_bytes",
+    "kw_sim_footprint_bytes", "kw_diag_copy",
 )
 
 
@@ -99,6 +99,8 @@ def load_library(path: str = LIB_PATH):
     lib.kw_sim_profile_get.restype = C.c_int
     lib.kw_sim_footprint_bytes.argtypes = [C.POINTER(KwSimDesc)]
     lib.kw_sim_footprint_bytes.restype = C.c_uint64
+    lib.kw_diag_copy.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+    lib.kw_diag_copy.restype = C.c_int
     _lib = lib
     return lib
 
@@ -316,3 +318,11 @@ def footprint_bytes(f: dict, record=("p",), record_start_index: int = 1) -> int:
     n = int(lib.kw_sim_footprint_bytes(C.byref(d)))
     del keep
     return n
+
+
+def diag_copy(nbytes: int, device: int = 0) -> float:
+    """Time one plain device copy of nbytes (counter calibration); returns ms."""
+    lib = load_library()
+    ms = C.c_double()
+    _check(lib, lib.kw_diag_copy(int(nbytes), device, C.byref(ms)))
+    return ms.value
