@@ -83,7 +83,7 @@ def build(verbose: bool = False, with_io: bool = True) -> dict:
             o = _obj(src)
             _run([HIPCC, *HIP_FLAGS, *inc, f"-I{h5inc}", "-c", os.path.join(CSRC, src), "-o", o], verbose)
             ioobjs.append(o)
-        rpath = f"-Wl,-rpath,$ORIGIN:{h5lib}:/opt/rocm/lib"
+        rpath = f"-Wl,-rpath,$ORIGIN:/opt/rocm/lib:/usr/lib/x86_64-linux-gnu:{h5lib}"
         _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *ioobjs, "-o", io + ".tmp",
               "-Wl,-soname,libkwave_io.so", f"-L{LIB}", "-lkwave_hip", f"-L{h5lib}", "-lhdf5", rpath], verbose)
         os.replace(io + ".tmp", io)
@@ -93,7 +93,7 @@ def build(verbose: bool = False, with_io: bool = True) -> dict:
             exe = os.path.join(BIN, "kspaceFirstOrder-HIP")
             _run([HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", *inc, main, "-o", exe + ".tmp",
                   f"-L{LIB}", "-lkwave_io", "-lkwave_hip",
-                  f"-Wl,-rpath,$ORIGIN/../lib:{h5lib}:/opt/rocm/lib"], verbose)
+                  f"-Wl,-rpath,$ORIGIN/../lib:/opt/rocm/lib:/usr/lib/x86_64-linux-gnu:{h5lib}"], verbose)
             os.replace(exe + ".tmp", exe)
             built["binary"] = exe
     shutil.rmtree(os.path.join(LIB, "obj"), ignore_errors=True)

@@ -551,6 +551,31 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
   if (d->absorbing_flag == 1 && std::fabs(d->alpha_power - 1.0) < 1e-12)
     return fail(KW_ERR_INVALID, "alpha_power = 1 is not supported by the power-law absorption model");
 
+  // host-side checks of everything the kernels index with, before any HIP call
+  const uint64_t psz_chk[3] = {d->pml_x_size, d->pml_y_size, d->Nz > 1 ? d->pml_z_size : 0};
+  const uint64_t n_chk[3] = {d->Nx, d->Ny, d->Nz};
+  for (int a = 0; a < 3; ++a)
+    if (2 * psz_chk[a] > n_chk[a])
+      return fail(KW_ERR_INVALID, "PML size %llu too large for an axis of %llu points",
+                  (unsigned long long)psz_chk[a], (unsigned long long)n_chk[a]);
+  if (d->sensor_npoints && !d->sensor_mask_index) return fail(KW_ERR_INVALID, "sensor_mask_index missing");
+  std::vector<long long> slin, plin, ulin;
+  if ((st = load_indices(d->sensor_mask_index, d->sensor_npoints, N, "sensor_mask_index", slin))) return st;
+  if (d->p_source_flag) {
+    if (d->p_source_mode > 2) return fail(KW_ERR_INVALID, "p_source_mode %llu", (unsigned long long)d->p_source_mode);
+    if (!d->p_source_index || !d->p_source_input || d->p_source_npoints == 0)
+      return fail(KW_ERR_INVALID, "p source: index or input missing");
+    if ((st = load_indices(d->p_source_index, d->p_source_npoints, N, "p_source_index", plin))) return st;
+  }
+  const uint64_t uflag_chk[3] = {d->ux_source_flag, d->uy_source_flag, d->Nz > 1 ? d->uz_source_flag : 0};
+  const float* uin_chk[3] = {d->ux_source_input, d->uy_source_input, d->uz_source_input};
+  if (uflag_chk[0] || uflag_chk[1] || uflag_chk[2]) {
+    if (d->u_source_mode > 2) return fail(KW_ERR_INVALID, "u_source_mode %llu", (unsigned long long)d->u_source_mode);
+    if (!d->u_source_index || d->u_source_npoints == 0) return fail(KW_ERR_INVALID, "u source: index missing");
+    for (int a = 0; a < 3; ++a)
+      if (uflag_chk[a] && !uin_chk[a]) return fail(KW_ERR_INVALID, "u source: input of component %d missing", a);
+    if ((st = load_indices(d->u_source_index, d->u_source_npoints, N, "u_source_index", ulin))) return st;
+  }
   kw_sim* s = new kw_sim();
   try {
     Grid& g = s->g;
@@ -637,12 +662,6 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     const uint64_t psz[3] = {d->pml_x_size, d->pml_y_size, d->pml_z_size};
     const double pal[3] = {d->pml_x_alpha, d->pml_y_alpha, d->pml_z_alpha};
     for (int a = 0; a < g.dim; ++a) {
-      if (2 * psz[a] > (uint64_t)nfull[a]) {
-        free_all(s);
-        delete s;
-        return fail(KW_ERR_INVALID, "PML size %llu too large for axis of %d points", (unsigned long long)psz[a],
-                    nfull[a]);
-      }
       float* pm = s->alloc<float>(nfull[a]);
       float* pms = s->alloc<float>(nfull[a]);
       launch_pml(st0, pm, nfull[a], dxs[a], d->dt, d->c_ref, (int)psz[a], pal[a], 0);
@@ -709,12 +728,6 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     if (d->p0_source_input) s->p0 = s->upload(d->p0_source_input, (size_t)N);
 
     // sensor
-    std::vector<long long> slin;
-    if ((st = load_indices(d->sensor_mask_index, d->sensor_npoints, N, "sensor_mask_index", slin))) {
-      free_all(s);
-      delete s;
-      return st;
-    }
     s->ns = (int)d->sensor_npoints;
     std::vector<int> srows;
     build_rows(slin, g, srows);
@@ -741,12 +754,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
 
     // sources (point lists share the row CSR construction)
     if (d->p_source_flag) {
-      std::vector<long long> lin;
-      if ((st = load_indices(d->p_source_index, d->p_source_npoints, N, "p_source_index", lin))) {
-        free_all(s);
-        delete s;
-        return st;
-      }
+      const std::vector<long long>& lin = plin;
       std::vector<int> rows;
       build_rows(lin, g, rows);
       int* rd = s->upload(rows.data(), rows.size());
@@ -761,12 +769,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     const uint64_t uf[3] = {d->ux_source_flag, d->uy_source_flag, g.dim == 3 ? d->uz_source_flag : 0};
     const float* uin[3] = {d->ux_source_input, d->uy_source_input, d->uz_source_input};
     if (uf[0] || uf[1] || uf[2]) {
-      std::vector<long long> lin;
-      if ((st = load_indices(d->u_source_index, d->u_source_npoints, N, "u_source_index", lin))) {
-        free_all(s);
-        delete s;
-        return st;
-      }
+      const std::vector<long long>& lin = ulin;
       std::vector<int> rows;
       build_rows(lin, g, rows);
       int* rd = s->upload(rows.data(), rows.size());

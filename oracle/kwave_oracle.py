@@ -320,6 +320,11 @@ class Spectral:
 # time loop
 # ---------------------------------------------------------------------------
 
+def nonlinear_density(srho, rho0):
+    """rho0_plus_rho = 2 (rhox + rhoy + rhoz) + rho0 (kspaceFirstOrder3D.m:920)."""
+    return 2.0 * srho + rho0
+
+
 def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: bool = False,
         nsteps: int | None = None) -> dict:
     """Run the time loop; return outputs named and shaped like the output file.
@@ -398,7 +403,7 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
             for a in range(dim):
                 rho[a] = pml[a] * (pml[a] * rho[a] - dt * rho0 * du[a])
         else:
-            rpr = 2.0 * sum(rho) + rho0
+            rpr = nonlinear_density(sum(rho), rho0)
             for a in range(dim):
                 rho[a] = pml[a] * (pml[a] * rho[a] - dt * rpr * du[a])
         # pressure source (3D.m:928-960)

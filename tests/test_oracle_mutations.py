@@ -1,0 +1,64 @@
+"""The oracle's pins are discriminating: plant a plausible mistake in the
+oracle (CPU only, via monkeypatch) and the pin that guards it must fail."""
+
+import numpy as np
+import pytest
+
+import test_oracle as pins
+
+
+def _fails(fn, *args):
+    with pytest.raises(AssertionError):
+        fn(*args)
+
+
+def test_dropped_kappa_breaks_closed_form(kw, oracle, monkeypatch):
+    monkeypatch.setattr(oracle, "sinc", lambda x: np.ones_like(np.asarray(x, dtype=float)))
+    _fails(pins.test_lossless_homogeneous_matches_closed_form, kw, oracle, (32, 24))
+
+
+def test_wrong_shift_sign_breaks_closed_form(kw, oracle, monkeypatch):
+    orig = oracle.Operators.__init__
+
+    def init(self, pb, full_complex=False):
+        orig(self, pb, full_complex)
+        self.d_pos = list(self.d_neg)
+
+    monkeypatch.setattr(oracle.Operators, "__init__", init)
+    _fails(pins.test_lossless_homogeneous_matches_closed_form, kw, oracle, (32, 24))
+
+
+def test_absorption_sign_breaks_attenuation(kw, oracle, monkeypatch):
+    orig = oracle.Operators.__init__
+
+    def init(self, pb, full_complex=False):
+        orig(self, pb, full_complex)
+        self.absorb_tau = -self.absorb_tau
+
+    monkeypatch.setattr(oracle.Operators, "__init__", init)
+    _fails(pins.test_power_law_attenuation, kw, oracle)
+
+
+def test_convective_factor_breaks_fubini(kw, oracle, monkeypatch):
+    monkeypatch.setattr(oracle, "nonlinear_density", lambda srho, rho0: 1.0 * srho + rho0)
+    _fails(pins.test_second_harmonic_follows_fubini, kw, oracle, 0.0, 0.05)
+
+
+def test_unstaggered_pml_breaks_closed_form(oracle, monkeypatch):
+    orig = oracle.get_pml
+    monkeypatch.setattr(oracle, "get_pml", lambda n, d, dt, c, s, a, st: orig(n, d, dt, c, s, a, False))
+    _fails(pins.test_pml_closed_form, oracle)
+
+
+def test_source_scaling_factor_breaks_amplitude(kw, oracle, monkeypatch):
+    fe = kw.frontend
+    orig = fe.save_to_disk_dict
+
+    def halved(*a, **k):
+        f = orig(*a, **k)
+        if "p_source_input" in f:
+            f["p_source_input"] = f["p_source_input"] * 0.5
+        return f
+
+    monkeypatch.setattr(fe, "save_to_disk_dict", halved)
+    _fails(pins.test_additive_source_scaling_against_finer_dt, kw, oracle)
