@@ -12,6 +12,7 @@ binary (kspaceFirstOrder3DC.m:544).
 
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -34,7 +35,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 
-SOLVER_SOURCES = ["kw_kernels.hip", "kw_sim.cpp"]
+SOLVER_SOURCES = ["kw_kernels.hip", "kw_pencil.hip", "kw_xrow.hip", "kw_sim.cpp"]
 IO_SOURCES = ["kw_io.cpp"]
 
 
@@ -64,11 +65,13 @@ def build(verbose: bool = False, with_io: bool = True) -> dict:
     os.makedirs(os.path.join(LIB, "obj"), exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
     inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
-    objs = []
-    for src in SOLVER_SOURCES:
-        o = _obj(src)
-        _run([HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", o], verbose)
-        objs.append(o)
+    objs = [_obj(src) for src in SOLVER_SOURCES]
+    # translation units compile in parallel (the length-specialised kernels dominate)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(objs), os.cpu_count() or 1)) as ex:
+        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+                for src, o in zip(SOLVER_SOURCES, objs)]
+        for fu in futs:
+            fu.result()
     solver = os.path.join(LIB, "libkwave_hip.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", solver + ".tmp",
           "-Wl,-soname,libkwave_hip.so"], verbose)

@@ -1,0 +1,221 @@
+// Compile-time specialised Stockham FFTs for the pencil and row kernels.
+//
+// The length N, the radix plan (greedy 16, 8, 4, 2, then 3, 5, 7), the number
+// of transforms per workgroup CNT and the thread count NT are template
+// parameters, so every index, stride and loop bound is a constant.  A
+// transform is a chain of stages; the first stage reads through a caller
+// functor (typically straight from HBM) and the last stage writes through a
+// caller functor (typically straight to HBM, or a spectral multiply into
+// LDS), with LDS buffers only between stages.  Stage s (radix R, stride NS =
+// product of the earlier radices), butterfly i in [0, N/R):
+//   v[r] = x[i + r N/R] * W_N^(sign r k N/(NS R)),  k = i % NS
+//   y[(i/NS) NS R + k + r NS] = DFT_R(v)[r]
+#pragma once
+#include "kw_fft.hpp"
+
+namespace kwt {
+
+constexpr int next_radix(int m) {
+  return (m % 16 == 0) ? 16 : (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2
+       : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : 0;
+}
+constexpr int num_stages(int n) {
+  int s = 0;
+  while (n > 1) {
+    const int r = next_radix(n);
+    if (r == 0) return -1;
+    n /= r;
+    ++s;
+  }
+  return s;
+}
+constexpr int radix_of(int n, int s) {
+  for (int i = 0; i < s; ++i) n /= next_radix(n);
+  return next_radix(n);
+}
+constexpr int ns_of(int n, int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) {
+    const int r = next_radix(n);
+    p *= r;
+    n /= r;
+  }
+  return p;
+}
+
+// radix-16 as 4 x 4 with internal twiddles W16^(n2 k1)
+__device__ __forceinline__ void bf16(float2* v, float sign) {
+  const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;  // cos/sin(pi/8)
+  const float r2 = 0.70710678118654752440f;
+  float2 y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    float2 q[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+    bf4(q, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = q[k1];
+  }
+  // W16^m, m = n2*k1 in {1,2,3,2,4,6,3,6,9}
+  y[1][1] = cmul(y[1][1], make_float2(c1, sign * s1));
+  y[1][2] = cmul(y[1][2], make_float2(r2, sign * r2));
+  y[1][3] = cmul(y[1][3], make_float2(s1, sign * c1));
+  y[2][1] = cmul(y[2][1], make_float2(r2, sign * r2));
+  y[2][2] = cmul_si(y[2][2], sign);
+  y[2][3] = cmul(y[2][3], make_float2(-r2, sign * r2));
+  y[3][1] = cmul(y[3][1], make_float2(s1, sign * c1));
+  y[3][2] = cmul(y[3][2], make_float2(-r2, sign * r2));
+  y[3][3] = cmul(y[3][3], make_float2(-c1, -sign * s1));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 q[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    bf4(q, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = q[k2];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void bfly(float2* v, float sign) {
+  if constexpr (R == 2) bf2(v);
+  else if constexpr (R == 3) bf3(v, sign);
+  else if constexpr (R == 4) bf4(v, sign);
+  else if constexpr (R == 5) bf5(v, sign);
+  else if constexpr (R == 7) bf7(v, sign);
+  else if constexpr (R == 8) bf8(v, sign);
+  else bf16(v, sign);
+}
+
+// LDS layouts with a pad every 16 positions (spreads the stride-R Stockham
+// writes over banks).
+template <int N>
+struct Pad {
+  static constexpr int P = N + N / 16;
+  __device__ static constexpr int at(int a) { return a + (a >> 4); }
+};
+
+// pencils: transform index t fastest (coalesced columns), position a slowest
+template <int N, int CNT>
+struct PencilLds {
+  float2* p;
+  __device__ float2& operator()(int t, int a) const { return p[Pad<N>::at(a) * CNT + t]; }
+  static constexpr int size = Pad<N>::P * CNT;
+};
+// rows: position a fastest
+template <int N, int CNT>
+struct RowLds {
+  float2* p;
+  __device__ float2& operator()(int t, int a) const { return p[t * Pad<N>::P + Pad<N>::at(a)]; }
+  static constexpr int size = Pad<N>::P * CNT;
+};
+
+// One stage over `cnt` transforms (cnt <= CMAX; TFAST requires cnt == CMAX).
+// TFAST: work item w -> (t = w % CMAX, i = w / CMAX), else (t = w / NB, i = w % NB).
+// Store functors receive (t, a, value, slot) with slot = it * R + r, a
+// compile-time constant after unrolling (lets callers keep outputs in registers).
+template <int N, int S, int CMAX, int NT, bool TFAST, typename Load, typename Store>
+__device__ __forceinline__ void stage(const float2* tw, float sign, int cnt, Load& ld, Store& st) {
+  constexpr int R = radix_of(N, S);
+  constexpr int NS = ns_of(N, S);
+  constexpr int NB = N / R;
+  constexpr int TMAX = NB * CMAX;
+  constexpr int TWSTEP = N / (NS * R);
+  constexpr int ITER = (TMAX + NT - 1) / NT;
+  const int total = NB * cnt;
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (w >= total) break;
+    int t, i;
+    if constexpr (TFAST) {
+      t = w % CMAX;
+      i = w / CMAX;
+    } else {
+      t = w / NB;
+      i = w % NB;
+    }
+    const int k = i % NS;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = ld(t, i + r * NB);
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 wv = tw[r * k * TWSTEP];
+        wv.y *= -sign;
+        v[r] = cmul(v[r], wv);
+      }
+    }
+    bfly<R>(v, sign);
+    const int j = (i / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) st(t, j + r * NS, v[r], it * R + r);
+  }
+}
+
+// Full transform: stage 0 loads through `first`, the last stage stores through
+// `last`, intermediate stages ping-pong between LDS accessors A and B (B is
+// only touched for three or more stages).  Intermediate stages are separated
+// by barriers; there is no trailing barrier.
+template <int N, int CMAX, int NT, bool TFAST, int S = 0, typename Load, typename Store, typename Lds>
+__device__ __forceinline__ void run(const float2* tw, float sign, int cnt, Load& first, Store& last, Lds A, Lds B) {
+  constexpr int NSTAGES = num_stages(N);
+  static_assert(NSTAGES > 0, "length must factor into 2, 3, 5, 7");
+  if constexpr (NSTAGES == 1) {
+    stage<N, 0, CMAX, NT, TFAST>(tw, sign, cnt, first, last);
+  } else if constexpr (S == 0) {
+    auto st = [&](int t, int a, float2 v, int) { A(t, a) = v; };
+    stage<N, 0, CMAX, NT, TFAST>(tw, sign, cnt, first, st);
+    __syncthreads();
+    run<N, CMAX, NT, TFAST, 1>(tw, sign, cnt, first, last, A, B);
+  } else if constexpr (S == NSTAGES - 1) {
+    auto ld = [&](int t, int a) { return A(t, a); };
+    stage<N, S, CMAX, NT, TFAST>(tw, sign, cnt, ld, last);
+  } else {
+    auto ld = [&](int t, int a) { return A(t, a); };
+    auto st = [&](int t, int a, float2 v, int) { B(t, a) = v; };
+    stage<N, S, CMAX, NT, TFAST>(tw, sign, cnt, ld, st);
+    __syncthreads();
+    run<N, CMAX, NT, TFAST, S + 1>(tw, sign, cnt, first, last, B, A);
+  }
+}
+
+// Number of output slots a thread owns in the last stage, and the (t, a)
+// position of a slot (false if the thread has no work item for it).
+template <int N, int CMAX, int NT>
+constexpr int last_slots() {
+  constexpr int S = num_stages(N) - 1;
+  constexpr int R = radix_of(N, S);
+  return ((N / R) * CMAX + NT - 1) / NT * R;
+}
+
+template <int N, int CMAX, int NT, bool TFAST>
+__device__ __forceinline__ bool last_pos(int cnt, int slot, int& t, int& a) {
+  constexpr int S = num_stages(N) - 1;
+  constexpr int R = radix_of(N, S);
+  constexpr int NS = ns_of(N, S);
+  constexpr int NB = N / R;
+  const int it = slot / R, r = slot % R;
+  const int w = it * NT + (int)threadIdx.x;
+  if (w >= NB * cnt) return false;
+  int i;
+  if constexpr (TFAST) {
+    t = w % CMAX;
+    i = w / CMAX;
+  } else {
+    t = w / NB;
+    i = w % NB;
+  }
+  const int k = i % NS;
+  a = (i / NS) * NS * R + k + r * NS;
+  return true;
+}
+
+// Block index remap so that consecutive logical tiles land on the same XCD
+// (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).
+__device__ __forceinline__ int xcd_swizzle(int b, int nb) {
+  const int per = nb / 8;
+  if (b >= per * 8) return b;
+  return (b % 8) * per + b / 8;
+}
+
+}  // namespace kwt

@@ -504,10 +504,6 @@ __global__ void __launch_bounds__(256) k_x_c2r(Grid g, FftAxis ax, const float2*
 // y pass (3D): complex FFT along y of X-domain arrays, tiles of tx kx columns
 // at one z.  blockIdx = (kx tile, z, field).
 // ---------------------------------------------------------------------------
-struct Fields3 {
-  float2* f[3];
-};
-
 __global__ void __launch_bounds__(256) k_ypass(Grid g, FftAxis ax, const float2* __restrict__ tw_g, Fields3 F,
                                                float sign, int TX) {
   extern __shared__ float2 smem[];
@@ -727,6 +723,7 @@ static inline unsigned x_blocks(const Grid& g, int P) { return (unsigned)((g.row
 
 void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, float2* const* f, int nf,
                   float sign, TileLaunch tl) {
+  if (launch_ypass_fast(s, g, ax.n, tw, f, nf, sign)) return;
   Fields3 F{{f[0], nf > 1 ? f[1] : nullptr, nf > 2 ? f[2] : nullptr}};
   dim3 grid((g.Nxh + tl.tx - 1) / tl.tx, g.Nz, nf);
   size_t sm = (size_t)g.Ny * sizeof(float2) * (1 + 2 * tl.tx);
@@ -735,6 +732,7 @@ void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2*
 
 void launch_last(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const LastArgs& a,
                  const SpecTables& T, TileLaunch tl) {
+  if (launch_last_fast(s, g, ax.n, tw, a, T)) return;
   const int Q = g.dim == 3 ? g.Nxh * g.Ny : g.Nxh;
   dim3 grid((Q + tl.tx - 1) / tl.tx);
   size_t sm = (size_t)ax.n * sizeof(float2) * (1 + 3 * tl.tx);
@@ -744,6 +742,7 @@ void launch_last(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* 
 void launch_x_velocity(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                        float2* const* G, float* const* u, const Src* src, const Sens* sens, int init,
                        XLaunch xl) {
+  if (launch_x_velocity_fast(s, g, tw, m, G, u, src, sens, init)) return;
   VelArgs A;
   for (int c = 0; c < 3; ++c) {
     A.G[c] = c < g.dim ? G[c] : nullptr;
@@ -760,6 +759,7 @@ void launch_x_velocity(hipStream_t s, const Grid& g, const FftAxis& ax, const fl
 void launch_x_density(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                       float2* const* G, float* const* rho, const Src& psrc, float* srho_out, float2* pk,
                       const Sens& ps, XLaunch xl) {
+  if (launch_x_density_fast(s, g, tw, m, G, rho, psrc, srho_out, pk, ps)) return;
   DensArgs A;
   for (int c = 0; c < 3; ++c) {
     A.G[c] = c < g.dim ? G[c] : nullptr;
@@ -776,24 +776,28 @@ void launch_x_density(hipStream_t s, const Grid& g, const FftAxis& ax, const flo
 void launch_x_pressure(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                        const float2* GA, const float2* GB, const float* srho, float2* pk, const Sens& ps,
                        XLaunch xl) {
+  if (launch_x_pressure_fast(s, g, tw, m, GA, GB, srho, pk, ps)) return;
   hipLaunchKernelGGL(k_x_pressure, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false),
                      s, g, ax, tw, m, GA, GB, srho, pk, ps, xl.pairs);
 }
 
 void launch_x_init(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                    const float* p0, float* const* rho, float2* pk, const Sens& ps, XLaunch xl) {
+  if (launch_x_init_fast(s, g, tw, m, p0, rho, pk, ps)) return;
   hipLaunchKernelGGL(k_x_init, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false), s,
                      g, ax, tw, m, p0, rho[0], rho[1], g.dim == 3 ? rho[2] : nullptr, pk, ps, xl.pairs);
 }
 
 void launch_x_source(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Src& src, int unit,
                      float2* out, XLaunch xl) {
+  if (launch_x_source_fast(s, g, tw, src, unit, out)) return;
   hipLaunchKernelGGL(k_x_source, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false), s,
                      g, ax, tw, src, unit, out, xl.pairs);
 }
 
 void launch_x_c2r(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const float2* in, float* out,
                   XLaunch xl) {
+  if (launch_x_c2r_fast(s, g, tw, in, out)) return;
   hipLaunchKernelGGL(k_x_c2r, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false), s, g,
                      ax, tw, in, out, xl.pairs);
 }

@@ -88,6 +88,10 @@ struct LastArgs {
   float scale;
 };
 
+struct Fields3 {
+  float2* f[3];
+};
+
 // Host-side launch wrappers (kw_kernels.hip).
 struct XLaunch {
   int pairs;    // packed row pairs per workgroup
@@ -97,6 +101,24 @@ struct TileLaunch {
   int tx;       // pencils per workgroup
   int threads;
 };
+
+// length-specialised pencil passes (kw_pencil.hip); false if n has no instantiation
+bool pencil_supported(int n);
+bool launch_ypass_fast(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign);
+bool launch_last_fast(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T);
+
+// length-specialised x-row kernels (kw_xrow.hip); false if Nx has no instantiation
+bool row_supported(int nx);
+bool launch_x_velocity_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
+                            float* const* u, const Src* src, const Sens* sens, int init);
+bool launch_x_density_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
+                           float* const* rho, const Src& psrc, float* srho, float2* pk, const Sens& ps);
+bool launch_x_pressure_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, const float2* GA,
+                            const float2* GB, const float* srho, float2* pk, const Sens& ps);
+bool launch_x_init_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, const float* p0,
+                        float* const* rho, float2* pk, const Sens& ps);
+bool launch_x_source_fast(hipStream_t s, const Grid& g, const float2* tw, const Src& src, int unit, float2* out);
+bool launch_x_c2r_fast(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out);
 
 void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, float2* const* f,
                   int nf, float sign, TileLaunch tl);

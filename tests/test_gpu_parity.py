@@ -103,3 +103,16 @@ def test_p0_equals_two_clicks_gpu(kw, oracle):
     b = kw.run(f2)["p"]
     assert np.max(np.abs(a - b)) < 1e-5 * np.abs(a).max()
     assert cases.rel_l2(a, oracle.run(f1)["p"]) < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 96), (48, 64, 32), (40, 32, 192)])
+def test_3d_specialised_pencil_lengths(kw, oracle, shape):
+    """Grids whose y/z lengths have compiled pencil kernels (others use the generic ones)."""
+    f = cases.case_3d(N=shape, Nt=30, nonlinear=True, source="p", pml=4)
+    compare(kw, oracle, f, ("p", "p_max", "u_final"), f"3d_pencil_{shape}")
+
+
+@pytest.mark.parametrize("shape", [(96, 160), (128, 128), (64, 256)])
+def test_2d_specialised_last_axis(kw, oracle, shape):
+    f = cases.case_2d_absorbing_psource(N=shape, Nt=60)
+    compare(kw, oracle, f, ("p", "p_rms"), f"2d_last_{shape}")
