@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=120)
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
+    ap.add_argument("--calib-copy-mib", type=int, default=1024,
                     help="also run one plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -85,9 +85,12 @@ def main():
     kw = import_pkg()
     from kwave_amd import configs
 
+    copy_gbs = None
     if args.calib_copy_mib:
         ms = kw.native.diag_copy(args.calib_copy_mib << 20)
-        print(f"calibration copy: {args.calib_copy_mib} MiB in {ms:.3f} ms", file=sys.stderr, flush=True)
+        copy_gbs = 2 * (args.calib_copy_mib << 20) / (ms * 1e-3) / 1e9
+        print(f"calibration copy: {args.calib_copy_mib} MiB in {ms:.3f} ms ({copy_gbs:.0f} GB/s read+write)",
+              file=sys.stderr, flush=True)
 
     K, W = args.steps, args.warmup
     f = configs.c2(Nt=max(K, W, 1))
@@ -154,6 +157,8 @@ def main():
                           "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": per_kernel,
     }
+    if copy_gbs:
+        out["copy_GBs"] = round(copy_gbs, 1)
 
     if not args.no_cpu:
         from oracle import kwave_oracle  # CPU baseline only

@@ -166,9 +166,10 @@ int kw_sim_profile_get(kw_sim* sim, const char** names, double* total_ms, uint64
  * Used for HBM-footprint admission by the serving loop. */
 uint64_t kw_sim_footprint_bytes(const kw_sim_desc* desc);
 
-/* Diagnostic with no counterpart in the reference: one plain device copy of
- * `bytes` bytes (fresh buffers), timed with HIP events.  Gives the
- * FETCH_SIZE/WRITE_SIZE counters a kernel of known traffic for calibration. */
+/* Diagnostic with no counterpart in the reference: plain device copies of
+ * `bytes` bytes (one untimed, then ten timed with HIP events; *ms is the mean
+ * per copy).  Gives the FETCH_SIZE/WRITE_SIZE counters a kernel of known
+ * traffic for calibration, and the copy bandwidth a practical ceiling. */
 kw_status kw_diag_copy(uint64_t bytes, int device, double* ms);
 
 #ifdef __cplusplus

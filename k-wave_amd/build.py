@@ -35,7 +35,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 
-SOLVER_SOURCES = ["kw_kernels.hip", "kw_pencil.hip", "kw_xrow.hip", "kw_sim.cpp"]
+SOLVER_SOURCES = ["kw_kernels.hip", "kw_sim.cpp"] + [f"kw_{k}_nt{n}.hip" for k in ("pencil", "xrow")
+                                                      for n in (64, 128, 256)]
 IO_SOURCES = ["kw_io.cpp"]
 
 

@@ -713,6 +713,32 @@ __global__ void k_finish_rms(long long n, const double* sum, double count, float
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+bool fast_length(int n) {
+  switch (n) {
+#define KW_CASE(n) case n:
+    KW_FAST_SIZES(KW_CASE)
+#undef KW_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+// dispatch to the length-specialised kernels instantiated for `nt` threads
+#define KW_BY_NT(nt, fn, ...)                                   \
+  ((nt) == 64 ? fn<64>(__VA_ARGS__)                             \
+              : (nt) == 128 ? fn<128>(__VA_ARGS__) : (nt) == 256 ? fn<256>(__VA_ARGS__) : false)
+
+static bool ypass_fast(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign,
+                       int nt) {
+  return KW_BY_NT(nt, launch_ypass_fast_t, s, g, n, tw, f, nf, sign);
+}
+
+static bool last_fast(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T,
+                      int nt) {
+  return KW_BY_NT(nt, launch_last_fast_t, s, g, n, tw, a, T);
+}
+
 static size_t x_smem(const Grid& g, int P, int nbuf, bool slot) {
   size_t s = (size_t)g.Nx * sizeof(float2) + (size_t)nbuf * P * g.Nx * sizeof(float2);
   if (slot) s += (size_t)2 * P * g.Nx * sizeof(int);
@@ -723,7 +749,7 @@ static inline unsigned x_blocks(const Grid& g, int P) { return (unsigned)((g.row
 
 void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, float2* const* f, int nf,
                   float sign, TileLaunch tl) {
-  if (launch_ypass_fast(s, g, ax.n, tw, f, nf, sign)) return;
+  if (ypass_fast(s, g, ax.n, tw, f, nf, sign, tl.nt)) return;
   Fields3 F{{f[0], nf > 1 ? f[1] : nullptr, nf > 2 ? f[2] : nullptr}};
   dim3 grid((g.Nxh + tl.tx - 1) / tl.tx, g.Nz, nf);
   size_t sm = (size_t)g.Ny * sizeof(float2) * (1 + 2 * tl.tx);
@@ -732,7 +758,7 @@ void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2*
 
 void launch_last(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const LastArgs& a,
                  const SpecTables& T, TileLaunch tl) {
-  if (launch_last_fast(s, g, ax.n, tw, a, T)) return;
+  if (last_fast(s, g, ax.n, tw, a, T, tl.nt)) return;
   const int Q = g.dim == 3 ? g.Nxh * g.Ny : g.Nxh;
   dim3 grid((Q + tl.tx - 1) / tl.tx);
   size_t sm = (size_t)ax.n * sizeof(float2) * (1 + 3 * tl.tx);
@@ -742,7 +768,7 @@ void launch_last(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* 
 void launch_x_velocity(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                        float2* const* G, float* const* u, const Src* src, const Sens* sens, int init,
                        XLaunch xl) {
-  if (launch_x_velocity_fast(s, g, tw, m, G, u, src, sens, init)) return;
+  if (KW_BY_NT(xl.nt, launch_x_velocity_fast_t, s, g, tw, m, G, u, src, sens, init)) return;
   VelArgs A;
   for (int c = 0; c < 3; ++c) {
     A.G[c] = c < g.dim ? G[c] : nullptr;
@@ -759,7 +785,7 @@ void launch_x_velocity(hipStream_t s, const Grid& g, const FftAxis& ax, const fl
 void launch_x_density(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                       float2* const* G, float* const* rho, const Src& psrc, float* srho_out, float2* pk,
                       const Sens& ps, XLaunch xl) {
-  if (launch_x_density_fast(s, g, tw, m, G, rho, psrc, srho_out, pk, ps)) return;
+  if (KW_BY_NT(xl.nt, launch_x_density_fast_t, s, g, tw, m, G, rho, psrc, srho_out, pk, ps)) return;
   DensArgs A;
   for (int c = 0; c < 3; ++c) {
     A.G[c] = c < g.dim ? G[c] : nullptr;
@@ -776,28 +802,28 @@ void launch_x_density(hipStream_t s, const Grid& g, const FftAxis& ax, const flo
 void launch_x_pressure(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                        const float2* GA, const float2* GB, const float* srho, float2* pk, const Sens& ps,
                        XLaunch xl) {
-  if (launch_x_pressure_fast(s, g, tw, m, GA, GB, srho, pk, ps)) return;
+  if (KW_BY_NT(xl.nt, launch_x_pressure_fast_t, s, g, tw, m, GA, GB, srho, pk, ps)) return;
   hipLaunchKernelGGL(k_x_pressure, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false),
                      s, g, ax, tw, m, GA, GB, srho, pk, ps, xl.pairs);
 }
 
 void launch_x_init(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                    const float* p0, float* const* rho, float2* pk, const Sens& ps, XLaunch xl) {
-  if (launch_x_init_fast(s, g, tw, m, p0, rho, pk, ps)) return;
+  if (KW_BY_NT(xl.nt, launch_x_init_fast_t, s, g, tw, m, p0, rho, pk, ps)) return;
   hipLaunchKernelGGL(k_x_init, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false), s,
                      g, ax, tw, m, p0, rho[0], rho[1], g.dim == 3 ? rho[2] : nullptr, pk, ps, xl.pairs);
 }
 
 void launch_x_source(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Src& src, int unit,
                      float2* out, XLaunch xl) {
-  if (launch_x_source_fast(s, g, tw, src, unit, out)) return;
+  if (KW_BY_NT(xl.nt, launch_x_source_fast_t, s, g, tw, src, unit, out)) return;
   hipLaunchKernelGGL(k_x_source, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false), s,
                      g, ax, tw, src, unit, out, xl.pairs);
 }
 
 void launch_x_c2r(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const float2* in, float* out,
                   XLaunch xl) {
-  if (launch_x_c2r_fast(s, g, tw, in, out)) return;
+  if (KW_BY_NT(xl.nt, launch_x_c2r_fast_t, s, g, tw, in, out)) return;
   hipLaunchKernelGGL(k_x_c2r, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false), s, g,
                      ax, tw, in, out, xl.pairs);
 }

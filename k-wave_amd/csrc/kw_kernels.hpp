@@ -94,31 +94,41 @@ struct Fields3 {
 
 // Host-side launch wrappers (kw_kernels.hip).
 struct XLaunch {
-  int pairs;    // packed row pairs per workgroup
-  int threads;  // workgroup size
+  int pairs;    // packed row pairs per workgroup (generic kernels)
+  int threads;  // workgroup size (generic kernels)
+  int nt;       // workgroup size of the length-specialised kernels (64/128/256; 0 = generic only)
 };
 struct TileLaunch {
-  int tx;       // pencils per workgroup
+  int tx;       // pencils per workgroup (generic kernels)
   int threads;
+  int nt;       // workgroup size of the length-specialised kernels (64/128/256; 0 = generic only)
 };
 
-// length-specialised pencil passes (kw_pencil.hip); false if n has no instantiation
-bool pencil_supported(int n);
-bool launch_ypass_fast(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign);
-bool launch_last_fast(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T);
+// Axis lengths with compiled length-specialised kernels (kw_pencil_nt*.hip,
+// kw_xrow_nt*.hip, instantiated for 64-, 128- and 256-thread workgroups).
+#define KW_FAST_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(160) X(192) X(256) X(384) X(512) X(768)
+bool fast_length(int n);
 
-// length-specialised x-row kernels (kw_xrow.hip); false if Nx has no instantiation
-bool row_supported(int nx);
-bool launch_x_velocity_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
-                            float* const* u, const Src* src, const Sens* sens, int init);
-bool launch_x_density_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
-                           float* const* rho, const Src& psrc, float* srho, float2* pk, const Sens& ps);
-bool launch_x_pressure_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, const float2* GA,
-                            const float2* GB, const float* srho, float2* pk, const Sens& ps);
-bool launch_x_init_fast(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, const float* p0,
-                        float* const* rho, float2* pk, const Sens& ps);
-bool launch_x_source_fast(hipStream_t s, const Grid& g, const float2* tw, const Src& src, int unit, float2* out);
-bool launch_x_c2r_fast(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out);
+template <int NT>
+bool launch_ypass_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign);
+template <int NT>
+bool launch_last_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T);
+template <int NT>
+bool launch_x_velocity_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
+                              float* const* u, const Src* src, const Sens* sens, int init);
+template <int NT>
+bool launch_x_density_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
+                             float* const* rho, const Src& psrc, float* srho, float2* pk, const Sens& ps);
+template <int NT>
+bool launch_x_pressure_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, const float2* GA,
+                              const float2* GB, const float* srho, float2* pk, const Sens& ps);
+template <int NT>
+bool launch_x_init_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, const float* p0,
+                          float* const* rho, float2* pk, const Sens& ps);
+template <int NT>
+bool launch_x_source_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Src& src, int unit, float2* out);
+template <int NT>
+bool launch_x_c2r_fast_t(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out);
 
 void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, float2* const* f,
                   int nf, float sign, TileLaunch tl);

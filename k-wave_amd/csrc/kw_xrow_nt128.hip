@@ -1,0 +1,6 @@
+// x-row kernels instantiated for 128-thread workgroups (templates in kw_xrow.inc).
+#include "kw_xrow.inc"
+
+namespace kw {
+KW_ROW_INSTANTIATE(128)
+}  // namespace kw
