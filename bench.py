@@ -9,8 +9,11 @@ on a separate instance first.
 
 Also reported: the roofline of the dominant kernel (algorithmic bytes per
 launch / measured average launch time, per-kernel HIP events on the solver's
-stream), the whole-step algorithmic bandwidth, and the FP64 CPU oracle timed
-on a bounded sample of the same workload.
+stream), the whole-step algorithmic bandwidth, the FP64 CPU oracle timed on a
+bounded sample of the same workload, and the serving leg: a sample of the C5
+request stream run by the plain path (one simulation at a time) and by the
+serving loop (continuous batching), with throughput and p50/p99 latency for
+burst and Poisson arrivals.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -67,6 +70,76 @@ def kernel_bytes(f: dict, kind: str, launches_per_step: dict) -> float:
     return 0.0
 
 
+def _updates(f: dict) -> int:
+    return int(np.prod([int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz", "Nt")]))
+
+
+def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int):
+    """Submit reqs at the given arrival offsets (s) to a fresh server; wait for all."""
+    with kw.Server(max_active=max_active, steps_per_quantum=quantum) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for f, ta in zip(reqs, arrivals):
+            wait = t0 + ta - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(srv.submit(f, record=("p",)))
+        for r in ids:
+            srv.wait(r)
+        makespan = time.perf_counter() - t0
+        infos = [srv.info(r) for r in ids]
+        bad = [(r, srv.error(r)) for r, i in zip(ids, infos) if i["state"] != "done"]
+        if bad:
+            raise RuntimeError(f"serving requests failed: {bad[:3]}")
+        outs = [srv.fetch(r, "p") for r in ids]
+        peak = srv.stats()["hbm_peak_reserved_bytes"]
+    lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    upd = sum(_updates(f) for f in reqs)
+    res = {"max_active": max_active, "makespan_s": round(makespan, 3), "updates_per_s": upd / makespan,
+           "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+           "hbm_peak_reserved_GiB": round(peak / 2 ** 30, 2)}
+    return res, outs
+
+
+def serving_leg(kw, M: int, rates, max_active: int, quantum: int) -> dict:
+    """C5 serving stream: plain (one simulation at a time) vs served (continuous
+    batching over streams) on the same requests, burst and Poisson arrivals."""
+    import concurrent.futures as cf
+    from kwave_amd import configs
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(8) as ex:
+        reqs = list(ex.map(configs.c5_request, range(M)))
+    gen_s = time.perf_counter() - t0
+    print(f"serving: generated {M} requests in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    kw.run(configs.c5_request(0, grids=((32, 32, 32),), nt_range=(8, 8)))  # code objects loaded
+    burst = [0.0] * M
+    plain, plain_out = _serve_stream(kw, reqs, burst, 1, quantum)
+    print(f"serving: plain burst {plain}", file=sys.stderr, flush=True)
+    served, served_out = _serve_stream(kw, reqs, burst, max_active, quantum)
+    print(f"serving: served burst {served}", file=sys.stderr, flush=True)
+    identical = all(np.array_equal(a, b) for a, b in zip(plain_out, served_out))
+    del plain_out, served_out
+    out = {"requests": M, "sample": f"c5_request(0..{M - 1}) of the seeded serving stream",
+           "updates": sum(_updates(f) for f in reqs), "steps_per_quantum": quantum,
+           "burst": {"plain": plain, "served": served,
+                     "speedup": round(served["updates_per_s"] / plain["updates_per_s"], 3)},
+           "identical_to_plain": bool(identical), "poisson": []}
+    capacity = M / plain["makespan_s"]  # plain-path requests per second
+    rng = np.random.default_rng(5999)
+    gaps = rng.exponential(1.0, M)
+    for r in rates:
+        lam = r * capacity
+        arr = list(np.cumsum(gaps / lam) - gaps[0] / lam)
+        p, _ = _serve_stream(kw, reqs, arr, 1, quantum)
+        s, _ = _serve_stream(kw, reqs, arr, max_active, quantum)
+        out["poisson"].append({"rate_per_s": round(lam, 3), "load_vs_plain_capacity": r, "plain": p, "served": s})
+        print(f"serving: poisson load {r}: plain p50/p99 {p['p50_latency_s']}/{p['p99_latency_s']} s, "
+              f"served {s['p50_latency_s']}/{s['p99_latency_s']} s", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +149,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--calib-copy-mib", type=int, default=1024,
                     help="also run one plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="C5 serving-stream sample size (0 skips the serving leg)")
+    ap.add_argument("--serving-rates", default="0.9",
+                    help="Poisson arrival rates as fractions of the plain path's capacity")
+    ap.add_argument("--serving-max-active", type=int, default=8)
+    ap.add_argument("--serving-quantum", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -159,6 +238,11 @@ def main():
     }
     if copy_gbs:
         out["copy_GBs"] = round(copy_gbs, 1)
+
+    if args.serving_requests > 0:
+        rates = [float(x) for x in args.serving_rates.split(",") if x]
+        out["serving"] = serving_leg(kw, args.serving_requests, rates, args.serving_max_active,
+                                     args.serving_quantum)
 
     if not args.no_cpu:
         from oracle import kwave_oracle  # CPU baseline only

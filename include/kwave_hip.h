@@ -142,6 +142,9 @@ kw_status kw_sim_sync(kw_sim* sim);
 /* Number of time steps completed so far. */
 uint64_t kw_sim_steps_done(const kw_sim* sim);
 
+/* The handle's own stream (a hipStream_t), used by kw_sim_run(..., NULL). */
+void* kw_sim_stream(kw_sim* sim);
+
 /* Byte size of an output (0 if not recorded). */
 size_t kw_sim_output_bytes(const kw_sim* sim, int output_id);
 
@@ -171,6 +174,74 @@ uint64_t kw_sim_footprint_bytes(const kw_sim_desc* desc);
  * per copy).  Gives the FETCH_SIZE/WRITE_SIZE counters a kernel of known
  * traffic for calibration, and the copy bandwidth a practical ceiling. */
 kw_status kw_diag_copy(uint64_t bytes, int device, double* ms);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop.  The reference runs one simulation per process launch
+ * (kspaceFirstOrder3DC.m:544, one system() call per kspaceFirstOrder3DC
+ * call; kWaveTester.m:1936-1950 loops over many such calls).  A server keeps
+ * many simulations resident on one device instead: requests are copied in at
+ * submit, admitted in arrival order when their device footprint
+ * (kw_sim_footprint_bytes) fits the HBM budget and a slot is free, advanced
+ * concurrently (each on its own stream), and their outputs are copied to
+ * host memory when they finish, after which their device memory is freed and
+ * the next request can be admitted.  A request's results are the same as
+ * those of kw_sim_create + kw_sim_run(Nt) + kw_sim_fetch on its own.
+ * ------------------------------------------------------------------------- */
+#define KW_ERR_TIMEOUT ((kw_status)6) /* kw_server_wait: not finished in time */
+
+typedef struct kw_server_config {
+  uint32_t abi_version;       /* KW_ABI_VERSION */
+  int32_t device;             /* HIP device ordinal */
+  uint64_t hbm_budget_bytes;  /* 0: 90% of the device memory free at create */
+  uint32_t max_active;        /* simulations resident at once; 0: 8 */
+  uint32_t steps_per_quantum; /* time steps queued per submission; 0: 16 */
+} kw_server_config;
+
+typedef enum kw_request_state {
+  KW_REQ_QUEUED = 0,
+  KW_REQ_RUNNING = 1,
+  KW_REQ_DONE = 2,
+  KW_REQ_FAILED = 3,
+  KW_REQ_UNKNOWN = 4 /* no such request (never submitted, or released) */
+} kw_request_state;
+
+/* Seconds are measured from kw_server_create on a monotonic clock. */
+typedef struct kw_request_info {
+  int32_t state;   /* kw_request_state */
+  int32_t status;  /* kw_status of a failed request, else KW_OK */
+  uint64_t footprint_bytes;
+  uint64_t point_updates; /* Nx*Ny*Nz*Nt */
+  double t_submit, t_admit, t_done;
+} kw_request_info;
+
+typedef struct kw_server_stats {
+  uint64_t submitted, completed, failed, queued, active;
+  uint64_t hbm_budget_bytes, hbm_reserved_bytes, hbm_peak_reserved_bytes;
+  uint64_t point_updates_done;
+  double elapsed_s;
+} kw_server_stats;
+
+typedef struct kw_server kw_server;
+
+kw_status kw_server_create(const kw_server_config* cfg, kw_server** out);
+/* Copies the description and every array it points to (the caller may free
+ * them on return) and queues it.  Fails at once (no id) for a description
+ * kw_sim_footprint_bytes rejects or whose footprint exceeds the budget. */
+kw_status kw_server_submit(kw_server* srv, const kw_sim_desc* desc, uint64_t* id);
+int kw_server_poll(kw_server* srv, uint64_t id); /* kw_request_state */
+/* Waits until the request is DONE or FAILED; timeout_s < 0 waits forever.
+ * KW_OK when finished (check the state), KW_ERR_TIMEOUT otherwise. */
+kw_status kw_server_wait(kw_server* srv, uint64_t id, double timeout_s);
+kw_status kw_server_request_info(kw_server* srv, uint64_t id, kw_request_info* info);
+/* Message of a FAILED request (empty otherwise); valid until release. */
+const char* kw_server_request_error(kw_server* srv, uint64_t id);
+size_t kw_server_output_bytes(kw_server* srv, uint64_t id, int output_id);
+kw_status kw_server_fetch(kw_server* srv, uint64_t id, int output_id, void* host_dst, size_t bytes);
+/* Drops a finished request and its host copies of the outputs. */
+kw_status kw_server_release(kw_server* srv, uint64_t id);
+kw_status kw_server_stats_get(kw_server* srv, kw_server_stats* stats);
+/* Fails queued requests, lets running ones finish, joins and frees. */
+kw_status kw_server_destroy(kw_server* srv);
 
 #ifdef __cplusplus
 }

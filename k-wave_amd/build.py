@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 
-SOLVER_SOURCES = ["kw_kernels.hip", "kw_sim.cpp"] + [f"kw_{k}_nt{n}.hip" for k in ("pencil", "xrow")
+SOLVER_SOURCES = ["kw_kernels.hip", "kw_sim.cpp", "kw_server.cpp", "kw_plane.hip"] + [f"kw_{k}_nt{n}.hip" for k in ("pencil", "xrow")
                                                       for n in (64, 128, 256)]
 IO_SOURCES = ["kw_io.cpp"]
 
@@ -75,7 +75,7 @@ def build(verbose: bool = False, with_io: bool = True) -> dict:
             fu.result()
     solver = os.path.join(LIB, "libkwave_hip.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", solver + ".tmp",
-          "-Wl,-soname,libkwave_hip.so"], verbose)
+          "-Wl,-soname,libkwave_hip.so", "-pthread"], verbose)
     os.replace(solver + ".tmp", solver)
     built = {"solver": solver}
     if with_io and hdf5_available() and all(os.path.exists(os.path.join(CSRC, s)) for s in IO_SOURCES):

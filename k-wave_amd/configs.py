@@ -87,3 +87,54 @@ def c4(Nt: int | None = None, N: int = 768) -> dict:
     medium = {"sound_speed": c, "density": rho, "alpha_coeff": alpha, "alpha_power": 1.1}
     return fe.save_to_disk_dict(kgrid, medium, {"p": sig, "p_mask": p_mask, "p_mode": "additive"},
                                 {"mask": sensor}, pml_size=pml)
+
+
+C5_GRIDS = ((64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 160, 192))
+C5_NT = (400, 1500)
+C5_RECORD = ("p",)
+C5_DESCRIPTION = ("serving stream: requests on 64^3, 96^3, 128^3 or 160x160x192 grids (PML 10, dx 0.1 mm), "
+                  "Nt uniform in [400, 1500], smooth heterogeneous c/rho/alpha (y=1.5), planar p tone burst "
+                  "at 0.5-1.5 MHz, 1% random binary sensor recording p")
+
+
+def smooth_field(rng, shape, corr_len: float, lo: float, hi: float) -> np.ndarray:
+    """White noise filtered by a Gaussian of std corr_len points, min-max rescaled to [lo, hi] (float32)."""
+    import scipy.fft as sfft
+
+    w = rng.standard_normal(shape, dtype=np.float32)
+    ks = [np.fft.fftfreq(n).astype(np.float32) for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1]).astype(np.float32)]
+    k2 = sum(k.reshape([-1 if a == i else 1 for a in range(len(shape))]) ** 2 for i, k in enumerate(ks))
+    g = np.exp(np.float32(-2.0 * math.pi ** 2 * corr_len ** 2) * k2)
+    f = sfft.irfftn(sfft.rfftn(w, workers=-1) * g, s=shape, workers=-1).astype(np.float32)
+    fmin, fmax = float(f.min()), float(f.max())
+    return (lo + (hi - lo) * (f - fmin) / (fmax - fmin)).astype(np.float32)
+
+
+def c5_request(i: int, grids=C5_GRIDS, nt_range=C5_NT, Nt: int | None = None) -> dict:
+    """Request i of the serving stream, drawn from default_rng(5000 + i).
+
+    Draw order: grid, Nt, source frequency, c, rho, alpha_coeff, sensor points.
+    Source: p on the whole y-z plane x = 11 (1-based), additive, unit-amplitude
+    3-cycle tone burst.  Sensor: round(0.01 N) distinct points, as a binary mask.
+    """
+    rng = np.random.default_rng(5000 + i)
+    shape = tuple(int(n) for n in grids[int(rng.integers(len(grids)))])
+    nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+    f0 = float(rng.uniform(0.5e6, 1.5e6))
+    c = smooth_field(rng, shape, 4.0, 1450.0, 1650.0)
+    rho = smooth_field(rng, shape, 4.0, 950.0, 1100.0)
+    alpha = smooth_field(rng, shape, 4.0, 0.002, 1.0)
+    N = int(np.prod(shape))
+    pts = np.sort(rng.choice(N, int(round(0.01 * N)), replace=False))
+    sensor = np.zeros(shape, dtype=np.float32)
+    sensor[np.unravel_index(pts, shape, order="F")] = 1
+    dx = 0.1e-3
+    kgrid = fe.KWaveGrid(shape[0], dx, shape[1], dx, shape[2], dx)
+    dt = 0.3 * dx / float(c.max())
+    kgrid.set_time(nt if Nt is None else int(Nt), dt)
+    p_mask = np.zeros(shape, dtype=np.float32)
+    p_mask[10, :, :] = 1                                                   # x = 11 (1-based)
+    sig = geo.tone_burst(1.0 / dt, f0, 3)
+    medium = {"sound_speed": c, "density": rho, "alpha_coeff": alpha, "alpha_power": 1.5}
+    return fe.save_to_disk_dict(kgrid, medium, {"p": sig, "p_mask": p_mask, "p_mode": "additive"},
+                                {"mask": sensor}, pml_size=10)

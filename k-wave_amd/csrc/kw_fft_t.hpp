@@ -179,6 +179,80 @@ __device__ __forceinline__ void run(const float2* tw, float sign, int cnt, Load&
   }
 }
 
+// One stage over all CMAX transforms that may write the buffer it reads: every
+// work item's inputs are loaded into registers, then a barrier, then the
+// butterflies and stores (a Stockham stage writes other positions than it
+// reads, so without the barrier a store could overwrite an unread input).
+template <int N, int S, int CMAX, int NT, bool TFAST, typename Load, typename Store>
+__device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld, Store& st) {
+  constexpr int R = radix_of(N, S);
+  constexpr int NS = ns_of(N, S);
+  constexpr int NB = N / R;
+  constexpr int TMAX = NB * CMAX;
+  constexpr int TWSTEP = N / (NS * R);
+  constexpr int ITER = (TMAX + NT - 1) / NT;
+  float2 v[ITER][R];
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (TMAX % NT == 0 || w < TMAX) {
+      const int t = TFAST ? w % CMAX : w / NB;
+      const int i = TFAST ? w / CMAX : w % NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = ld(t, i + r * NB);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (TMAX % NT == 0 || w < TMAX) {
+      const int t = TFAST ? w % CMAX : w / NB;
+      const int i = TFAST ? w / CMAX : w % NB;
+      const int k = i % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 wv = tw[r * k * TWSTEP];
+          wv.y *= -sign;
+          v[it][r] = cmul(v[it][r], wv);
+        }
+      }
+      bfly<R>(v[it], sign);
+      const int j = (i / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) st(t, j + r * NS, v[it][r], it * R + r);
+    }
+  }
+}
+
+// Full transform of all CMAX transforms in one LDS buffer L (in place):
+// stage 0 loads through `first`, the last stage stores through `last`, the
+// others read and write L.  Stages are separated by barriers; there is no
+// trailing barrier.
+template <int N, int CMAX, int NT, bool TFAST, int S = 0, typename Load, typename Store, typename Lds>
+__device__ __forceinline__ void run_ip(const float2* tw, float sign, Load& first, Store& last, Lds L) {
+  constexpr int NSTAGES = num_stages(N);
+  static_assert(NSTAGES > 0, "length must factor into 2, 3, 5, 7");
+  if constexpr (NSTAGES == 1) {
+    stage_ip<N, 0, CMAX, NT, TFAST>(tw, sign, first, last);
+  } else if constexpr (S == 0) {
+    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    stage_ip<N, 0, CMAX, NT, TFAST>(tw, sign, first, st);
+    __syncthreads();
+    run_ip<N, CMAX, NT, TFAST, 1>(tw, sign, first, last, L);
+  } else if constexpr (S == NSTAGES - 1) {
+    auto ld = [&](int t, int a) { return L(t, a); };
+    stage_ip<N, S, CMAX, NT, TFAST>(tw, sign, ld, last);
+  } else {
+    auto ld = [&](int t, int a) { return L(t, a); };
+    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    stage_ip<N, S, CMAX, NT, TFAST>(tw, sign, ld, st);
+    __syncthreads();
+    run_ip<N, CMAX, NT, TFAST, S + 1>(tw, sign, first, last, L);
+  }
+}
+
 // Number of output slots a thread owns in the last stage, and the (t, a)
 // position of a slot (false if the thread has no work item for it).
 template <int N, int CMAX, int NT>

@@ -660,10 +660,18 @@ __global__ void k_spectral_tables(Grid g, OpParams p, float* kappa, float* skapp
   const long long nspec = (long long)g.Nxh * g.Ny * g.Nz;
   const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nspec) return;
-  const int kxi = (int)(id % g.Nxh);
-  const long long rest = id / g.Nxh;
-  const int kyi = (int)(rest % g.Ny);
-  const int kzi = (int)(rest / g.Ny);
+  int kxi, kyi, kzi;
+  if (g.kxm) {  // [kx][z][y]
+    kxi = (int)(id / ((long long)g.Ny * g.Nz));
+    const long long rest = id - (long long)kxi * g.Ny * g.Nz;
+    kzi = (int)(rest / g.Ny);
+    kyi = (int)(rest - (long long)kzi * g.Ny);
+  } else {      // [z][y][kx]
+    kxi = (int)(id % g.Nxh);
+    const long long rest = id / g.Nxh;
+    kyi = (int)(rest % g.Ny);
+    kzi = (int)(rest / g.Ny);
+  }
   const double kx = kvec(kxi, g.Nx, p.dx[0]);
   const double ky = kvec(kyi, g.Ny, p.dx[1]);
   const double kz = g.dim == 3 ? kvec(kzi, g.Nz, p.dx[2]) : 0.0;

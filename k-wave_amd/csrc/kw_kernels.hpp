@@ -15,6 +15,9 @@ struct Grid {
   int Nx, Ny, Nz, Nxh, dim;
   int rows;  // Ny * Nz
   long long N;
+  // x-domain spectra layout: 0 row-major [z][y][kx] (kx fastest),
+  // 1 kx-major [kx][z][y] (plane pass, y fastest)
+  int kxm;
 };
 
 // Scalar-or-field value: field pointer (nullptr for a scalar) + scalar.
@@ -108,6 +111,11 @@ struct TileLaunch {
 // kw_xrow_nt*.hip, instantiated for 64-, 128- and 256-thread workgroups).
 #define KW_FAST_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(160) X(192) X(256) X(384) X(512) X(768)
 bool fast_length(int n);
+
+// plane pass (kw_plane.hip): LDS bytes for a (Ny, Nz) plane, 0 if none
+size_t plane_lds_bytes(int ny, int nz);
+bool launch_plane(hipStream_t s, const Grid& g, const float2* twY, const float2* twZ, const LastArgs& a,
+                  const SpecTables& T);
 
 template <int NT>
 bool launch_ypass_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign);

@@ -32,6 +32,16 @@ kw_status fail(kw_status st, const char* fmt, ...) {
   return st;
 }
 
+}  // namespace
+
+// shared with the serving loop (kw_server.cpp), which reports through the
+// same thread-local message
+namespace kw {
+void set_last_error(const char* msg) { g_err = msg ? msg : ""; }
+}  // namespace kw
+
+namespace {
+
 struct HipError {
   hipError_t e;
   const char* what;
@@ -73,9 +83,9 @@ bool make_axis(int n, FftAxis* ax) {
   return m == 1;
 }
 
-enum KernelKind { KK_YPASS, KK_LAST, KK_XVEL, KK_XDENS, KK_XPRES, KK_XINIT, KK_XSRC, KK_COUNT };
+enum KernelKind { KK_YPASS, KK_LAST, KK_XVEL, KK_XDENS, KK_XPRES, KK_XINIT, KK_XSRC, KK_PLANE, KK_COUNT };
 const char* kKindNames[KK_COUNT] = {"y_pass", "last_axis", "x_velocity", "x_density",
-                                    "x_pressure", "x_init", "x_source"};
+                                    "x_pressure", "x_init", "x_source", "yz_plane"};
 
 struct SrcData {
   uint64_t flag = 0;  // signal length
@@ -97,6 +107,7 @@ struct kw_sim {
   long long Nt = 0, t = 0, start = 1;
   uint64_t rec = 0;
   hipStream_t stream = nullptr;
+  hipStream_t last = nullptr;  // stream of the latest kw_sim_run
   FftAxis axX{}, axY{}, axL{};
   float2 *twX = nullptr, *twY = nullptr, *twL = nullptr;
   XLaunch xl{};
@@ -134,7 +145,9 @@ struct kw_sim {
   T_* alloc(size_t count) {
     void* p = nullptr;
     size_t bytes = std::max<size_t>(count * sizeof(T_), 16);
-    hipError_t e = hipMalloc(&p, bytes);
+    // stream-ordered: no device-wide synchronisation, so simulations created
+    // or destroyed while others run do not stall them
+    hipError_t e = hipMallocAsync(&p, bytes, stream);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       throw OomError{bytes};
@@ -145,7 +158,9 @@ struct kw_sim {
   template <typename T_>
   T_* upload(const T_* host, size_t count) {
     T_* d = alloc<T_>(count);
-    HIPCHK(hipMemcpy(d, host, count * sizeof(T_), hipMemcpyHostToDevice));
+    // host buffers may be temporaries of the caller: wait for this stream only
+    HIPCHK(hipMemcpyAsync(d, host, count * sizeof(T_), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     return d;
   }
   void zero(void* p, size_t bytes) { HIPCHK(hipMemsetAsync(p, 0, bytes, stream)); }
@@ -156,7 +171,18 @@ struct kw_sim {
 namespace {
 
 void free_all(kw_sim* s) {
-  for (void* p : s->owned) (void)hipFree(p);
+  // after the last work queued on the stream the handle last ran on
+  hipStream_t st = s->last ? s->last : s->stream;
+  if (st && st != s->stream && s->stream) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess) {
+      (void)hipEventRecord(e, st);
+      (void)hipStreamWaitEvent(s->stream, e, 0);
+      (void)hipEventDestroy(e);
+    }
+  }
+  for (void* p : s->owned) (void)hipFreeAsync(p, s->stream ? s->stream : st);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
   s->owned.clear();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -279,6 +305,20 @@ void last(kw_sim* s, hipStream_t st, const LastArgs& a) {
   launch_last(st, s->g, s->axL, s->twL, a, s->T, s->tll);
 }
 
+// Spectral operators on x-domain spectra: out_o = ifft_yz(op_o .* fft_yz(in_src(o))).
+// kx-major spectra: one plane pass; otherwise y pass over the inputs, the
+// last-axis pass, y pass over the outputs.
+void spectral(kw_sim* s, hipStream_t st, const LastArgs& a) {
+  if (s->g.kxm) {
+    Prof pr(s, st, KK_PLANE);
+    if (!launch_plane(st, s->g, s->twY, s->twL, a, s->T)) throw HipError{hipErrorInvalidConfiguration, "launch_plane"};
+    return;
+  }
+  ypass(s, st, a.in, a.nin, -1.f);
+  last(s, st, a);
+  ypass(s, st, a.out, a.nout, 1.f);
+}
+
 // Corrected additive source: real(ifft(source_kappa .* fft(scatter))) in X
 // domain, (3D.m:815-822); unit scatters 1.0 (pattern build).
 void source_spectrum(kw_sim* s, hipStream_t st, const Src& src, int unit, float2* out) {
@@ -286,8 +326,6 @@ void source_spectrum(kw_sim* s, hipStream_t st, const Src& src, int unit, float2
     Prof pr(s, st, KK_XSRC);
     launch_x_source(st, s->g, s->axX, s->twX, src, unit, out, s->xl);
   }
-  float2* f[1] = {out};
-  ypass(s, st, f, 1, -1.f);
   LastArgs a{};
   a.in[0] = out;
   a.out[0] = out;
@@ -295,8 +333,7 @@ void source_spectrum(kw_sim* s, hipStream_t st, const Src& src, int unit, float2
   a.nout = 1;
   a.outs[0] = SpecOut{0, OP_SKAPPA, 0};
   a.scale = (float)(1.0 / (double)s->g.N);
-  last(s, st, a);
-  ypass(s, st, f, 1, 1.f);
+  spectral(s, st, a);
 }
 
 Src make_src(const kw_sim* s, const SrcData& d, long long t) {
@@ -371,8 +408,6 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
       Prof pr(s, st, KK_XINIT);
       launch_x_init(st, g, s->axX, s->twX, s->m, s->p0, s->rho, s->Pk, ps, s->xl);
     }
-    float2* pk1[1] = {s->Pk};
-    ypass(s, st, pk1, 1, -1.f);
     LastArgs a{};
     a.in[0] = s->Pk;
     a.nin = 1;
@@ -382,8 +417,7 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
       a.outs[c] = SpecOut{0, OP_DPOS_K, c};
     }
     a.scale = invN;
-    last(s, st, a);
-    ypass(s, st, s->G, dim, 1.f);
+    spectral(s, st, a);
     {
       Prof pr(s, st, KK_XVEL);
       launch_x_velocity(st, g, s->axX, s->twX, s->m, s->G, s->u, nosrc, us, 1, s->xl);
@@ -399,8 +433,6 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
   }
 
   // ---- velocity: u = pml_sg (pml_sg u - dt rho_sg^-1 real(ifft(D+ kappa p_k))) (3D.m:790-801)
-  float2* pk1[1] = {s->Pk};
-  ypass(s, st, pk1, 1, -1.f);
   {
     LastArgs a{};
     a.in[0] = s->Pk;
@@ -411,9 +443,8 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
       a.outs[c] = SpecOut{0, OP_DPOS_K, c};
     }
     a.scale = invN;
-    last(s, st, a);
+    spectral(s, st, a);
   }
-  ypass(s, st, s->G, dim, 1.f);
   Src us_src[3] = {};
   for (int c = 0; c < dim; ++c) {
     us_src[c] = make_src(s, s->usrc[c], t);
@@ -425,7 +456,6 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
   }
 
   // ---- divergence: du_i = real(ifft(D-_i kappa fft(u_i))) (3D.m:905-907)
-  ypass(s, st, s->G, dim, -1.f);
   {
     LastArgs a{};
     a.nin = dim;
@@ -436,9 +466,8 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
       a.outs[c] = SpecOut{c, OP_DNEG_K, c};
     }
     a.scale = invN;
-    last(s, st, a);
+    spectral(s, st, a);
   }
-  ypass(s, st, s->G, dim, 1.f);
 
   // ---- density, pressure source, equation of state (3D.m:910-1017)
   Src psrc = make_src(s, s->psrc, t);
@@ -448,7 +477,6 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
     launch_x_density(st, g, s->axX, s->twX, s->m, s->G, s->rho, psrc, s->srho, s->Pk, ps, s->xl);
   }
   if (s->m.absorbing == 1) {
-    ypass(s, st, s->G, 2, -1.f);
     LastArgs a{};
     a.nin = 2;
     a.nout = 2;
@@ -457,8 +485,7 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
     a.outs[0] = SpecOut{0, OP_NABLA1, 0};
     a.outs[1] = SpecOut{1, OP_NABLA2, 0};
     a.scale = invN;
-    last(s, st, a);
-    ypass(s, st, s->G, 2, 1.f);
+    spectral(s, st, a);
     Prof pr(s, st, KK_XPRES);
     launch_x_pressure(st, g, s->axX, s->twX, s->m, s->G[0], s->G[1], s->srho, s->Pk, ps, s->xl);
   }
@@ -529,6 +556,17 @@ uint64_t kw_sim_footprint_bytes(const kw_sim_desc* d) {
   if (d->record_flags & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL | KW_REC_P_FINAL)) b += 3 * N * 4;
   if (d->record_flags & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL)) b += 2 * dim * N * 4;
   b += 8 * ns * 16;
+  // sources: signals, and the corrected pattern / spectrum of additive ones
+  auto src = [&](uint64_t flag, uint64_t mode, uint64_t many, uint64_t np) {
+    if (!flag) return (uint64_t)0;
+    uint64_t c = flag * (many ? np : 1) * 4;
+    if (mode == 2) c += nspec * 8 + (many ? 0 : N * 4);
+    return c;
+  };
+  b += src(d->p_source_flag, d->p_source_mode, d->p_source_many, d->p_source_npoints);
+  b += src(d->ux_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
+  b += src(d->uy_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
+  if (dim == 3) b += src(d->uz_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
   return b;
 }
 
@@ -625,6 +663,16 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     }
 
     HIPCHK(hipSetDevice(s->device));
+    // kx-major spectra and the plane pass when the (y, z) plane of one kx
+    // fits a workgroup's LDS and the x rows have specialised kernels
+    {
+      int lds_max = 0;
+      HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
+      const size_t need = g.dim == 3 ? plane_lds_bytes(g.Ny, g.Nz) : 0;
+      const char* pe = std::getenv("KW_PLANE");
+      const bool allow = !(pe && pe[0] == '0');
+      g.kxm = (allow && need > 0 && need <= (size_t)lds_max && s->xl.nt > 0) ? 1 : 0;
+    }
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&s->ev0));
     HIPCHK(hipEventCreate(&s->ev1));
@@ -834,6 +882,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
 kw_status kw_sim_run(kw_sim* s, uint64_t nsteps, void* stream) {
   if (!s) return fail(KW_ERR_INVALID, "null handle");
   hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  s->last = st;
   try {
     HIPCHK(hipSetDevice(s->device));
     const long long end = std::min<long long>(s->Nt, s->t + (long long)nsteps);
@@ -848,12 +897,14 @@ kw_status kw_sim_run(kw_sim* s, uint64_t nsteps, void* stream) {
 
 kw_status kw_sim_sync(kw_sim* s) {
   if (!s) return fail(KW_ERR_INVALID, "null handle");
-  hipError_t e = hipDeviceSynchronize();
+  hipError_t e = hipStreamSynchronize(s->last ? s->last : s->stream);
   if (e != hipSuccess) return fail(KW_ERR_DEVICE, "HIP error %d (%s)", (int)e, hipGetErrorString(e));
   return KW_OK;
 }
 
 uint64_t kw_sim_steps_done(const kw_sim* s) { return s ? (uint64_t)s->t : 0; }
+
+void* kw_sim_stream(kw_sim* s) { return s ? (void*)s->stream : nullptr; }
 
 static const void* output_ptr(const kw_sim* s, int id, size_t* bytes, bool* is_rms) {
   *is_rms = false;
@@ -904,7 +955,7 @@ kw_status kw_sim_fetch(kw_sim* s, int id, void* dst, size_t bytes) {
   if (bytes != need) return fail(KW_ERR_INVALID, "output %d needs %zu bytes, got %zu", id, need, bytes);
   try {
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(s->last ? s->last : s->stream));
     if (rms) {
       const long long nrec_done = std::max<long long>(0, s->t - s->start + 1);
       std::vector<double> h(2 * (size_t)s->ns);
@@ -924,7 +975,6 @@ kw_status kw_sim_fetch(kw_sim* s, int id, void* dst, size_t bytes) {
 kw_status kw_sim_destroy(kw_sim* s) {
   if (!s) return KW_OK;
   (void)hipSetDevice(s->device);
-  (void)hipDeviceSynchronize();
   free_all(s);
   delete s;
   return KW_OK;

@@ -32,8 +32,14 @@ _OUT_U_BASE = {"": 7, "_max": 10, "_min": 13, "_rms": 16, "_max_all": 19, "_min_
 EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
     "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
-    "kw_sim_footprint_bytes", "kw_diag_copy",
+    "kw_sim_footprint_bytes", "kw_diag_copy", "kw_sim_stream",
+    "kw_server_create", "kw_server_submit", "kw_server_poll", "kw_server_wait", "kw_server_request_info",
+    "kw_server_request_error", "kw_server_output_bytes", "kw_server_fetch", "kw_server_release",
+    "kw_server_stats_get", "kw_server_destroy",
 )
+
+KW_ERR_TIMEOUT = 6
+REQUEST_STATES = ("queued", "running", "done", "failed", "unknown")
 
 
 class KwRealArray(C.Structure):
@@ -63,6 +69,24 @@ class KwSimDesc(C.Structure):
         ("sensor_npoints", C.c_uint64), ("sensor_mask_index", C.POINTER(C.c_uint64)),
         ("record_flags", C.c_uint64), ("record_start_index", C.c_uint64),
     ]
+
+
+class KwServerConfig(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
+                ("max_active", C.c_uint32), ("steps_per_quantum", C.c_uint32)]
+
+
+class KwRequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("footprint_bytes", C.c_uint64),
+                ("point_updates", C.c_uint64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class KwServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("queued", C.c_uint64), ("active", C.c_uint64), ("hbm_budget_bytes", C.c_uint64),
+                ("hbm_reserved_bytes", C.c_uint64), ("hbm_peak_reserved_bytes", C.c_uint64),
+                ("point_updates_done", C.c_uint64), ("elapsed_s", C.c_double)]
 
 
 _lib = None
@@ -101,6 +125,24 @@ def load_library(path: str = LIB_PATH):
     lib.kw_sim_footprint_bytes.restype = C.c_uint64
     lib.kw_diag_copy.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
     lib.kw_diag_copy.restype = C.c_int
+    lib.kw_sim_stream.argtypes = [C.c_void_p]
+    lib.kw_sim_stream.restype = C.c_void_p
+    vp, u64 = C.c_void_p, C.c_uint64
+    for name, args, res in (
+            ("kw_server_create", [C.POINTER(KwServerConfig), C.POINTER(vp)], C.c_int),
+            ("kw_server_submit", [vp, C.POINTER(KwSimDesc), C.POINTER(u64)], C.c_int),
+            ("kw_server_poll", [vp, u64], C.c_int),
+            ("kw_server_wait", [vp, u64, C.c_double], C.c_int),
+            ("kw_server_request_info", [vp, u64, C.POINTER(KwRequestInfo)], C.c_int),
+            ("kw_server_request_error", [vp, u64], C.c_char_p),
+            ("kw_server_output_bytes", [vp, u64, C.c_int], C.c_size_t),
+            ("kw_server_fetch", [vp, u64, C.c_int, vp, C.c_size_t], C.c_int),
+            ("kw_server_release", [vp, u64], C.c_int),
+            ("kw_server_stats_get", [vp, C.POINTER(KwServerStats)], C.c_int),
+            ("kw_server_destroy", [vp], C.c_int)):
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
     _lib = lib
     return lib
 
@@ -326,3 +368,93 @@ def diag_copy(nbytes: int, device: int = 0) -> float:
     ms = C.c_double()
     _check(lib, lib.kw_diag_copy(int(nbytes), device, C.byref(ms)))
     return ms.value
+
+
+class Server:
+    """Serving loop (kw_server handle): submit many simulations, each admitted
+    by its device footprint and run concurrently with the others."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_active: int = 8,
+                 steps_per_quantum: int = 16):
+        self.lib = load_library()
+        cfg = KwServerConfig(KW_ABI_VERSION, device, int(hbm_budget_bytes), int(max_active),
+                             int(steps_per_quantum))
+        h = C.c_void_p()
+        _check(self.lib, self.lib.kw_server_create(C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._shape = {}
+
+    def submit(self, f: dict, record=("p",), record_start_index: int = 1) -> int:
+        d, keep = build_desc(f, record, record_start_index)
+        rid = C.c_uint64()
+        _check(self.lib, self.lib.kw_server_submit(self.handle, C.byref(d), C.byref(rid)))
+        del keep  # the server copied every array
+        dim = 3 if int(d.Nz) > 1 else 2
+        nrec = int(d.Nt) - max(1, int(record_start_index)) + 1
+        self._shape[rid.value] = (tuple(record), dim, int(d.sensor_npoints), nrec,
+                                  (int(d.Nz), int(d.Ny), int(d.Nx)))
+        return rid.value
+
+    def state(self, rid: int) -> str:
+        return REQUEST_STATES[self.lib.kw_server_poll(self.handle, rid)]
+
+    def wait(self, rid: int, timeout: float = -1.0) -> bool:
+        st = self.lib.kw_server_wait(self.handle, rid, float(timeout))
+        if st == KW_ERR_TIMEOUT:
+            return False
+        _check(self.lib, st)
+        return True
+
+    def info(self, rid: int) -> dict:
+        i = KwRequestInfo()
+        _check(self.lib, self.lib.kw_server_request_info(self.handle, rid, C.byref(i)))
+        return {"state": REQUEST_STATES[i.state], "status": i.status, "footprint_bytes": i.footprint_bytes,
+                "point_updates": i.point_updates, "t_submit": i.t_submit, "t_admit": i.t_admit,
+                "t_done": i.t_done}
+
+    def error(self, rid: int) -> str:
+        return (self.lib.kw_server_request_error(self.handle, rid) or b"").decode(errors="replace")
+
+    def fetch(self, rid: int, name: str) -> np.ndarray:
+        record, dim, ns, nrec, shape = self._shape[rid]
+        oid = output_id(name)
+        nbytes = self.lib.kw_server_output_bytes(self.handle, rid, oid)
+        if nbytes == 0:
+            raise KWaveError(1, f"output {name} of request {rid} is not available ({self.state(rid)})")
+        out = np.empty(nbytes // 4, dtype=np.float32)
+        _check(self.lib, self.lib.kw_server_fetch(self.handle, rid, oid, out.ctypes.data, nbytes))
+        if name in ("p", "ux", "uy", "uz"):
+            return out.reshape(nrec, ns)
+        if name.endswith("_all") or name.endswith("_final"):
+            return out.reshape(shape)
+        return out
+
+    def outputs(self, rid: int) -> dict:
+        record, dim = self._shape[rid][:2]
+        return {n: self.fetch(rid, n) for n in output_names(record, dim)}
+
+    def release(self, rid: int):
+        _check(self.lib, self.lib.kw_server_release(self.handle, rid))
+        self._shape.pop(rid, None)
+
+    def stats(self) -> dict:
+        s = KwServerStats()
+        _check(self.lib, self.lib.kw_server_stats_get(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in KwServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.kw_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -116,3 +116,41 @@ def test_3d_specialised_pencil_lengths(kw, oracle, shape):
 def test_2d_specialised_last_axis(kw, oracle, shape):
     f = cases.case_2d_absorbing_psource(N=shape, Nt=60)
     compare(kw, oracle, f, ("p", "p_rms"), f"2d_last_{shape}")
+
+
+def _uses_plane(kw, f):
+    with kw.Simulation(f, record=("p",)) as sim:
+        sim.set_profiling(True)
+        sim.run(2)
+        sim.sync()
+        return sim.profile().get("yz_plane", (0.0, 0))[1] > 0
+
+
+PLANE_CASES = [
+    ((48, 32, 32), dict(source="p", many=True, nonlinear=True)),     # corrected source spectra per step
+    ((32, 64, 64), dict(source="uxyz")),                             # single-signal additive patterns
+    ((64, 48, 48), dict(p0=True, source=None, absorbing=False)),     # p0 start chain
+    ((32, 96, 96), dict(source="ux", mode="dirichlet")),
+    ((32, 128, 128), dict(source="p")),
+    ((48, 64, 128), dict(source="ux", alpha_power=1.2)),
+    ((32, 128, 96), dict(source="p", mode="additive-no-correction")),
+]
+
+
+@pytest.mark.parametrize("shape,kwargs", PLANE_CASES, ids=[str(c[0]) for c in PLANE_CASES])
+def test_3d_plane_pass(kw, oracle, shape, kwargs):
+    """Grids whose (y, z) plane fits in LDS: kx-major spectra and the plane pass."""
+    f = cases.case_3d(N=shape, Nt=30, pml=4, **kwargs)
+    assert _uses_plane(kw, f), shape
+    compare(kw, oracle, f, ("p", "p_max", "u_final"), f"3d_plane_{shape}")
+
+
+def test_plane_pass_matches_three_pass(kw, monkeypatch):
+    """The same simulation with the plane pass and with y pass / last axis / y pass."""
+    f = cases.case_3d(N=(64, 64, 64), Nt=40, pml=6, source="uxyz", nonlinear=True)
+    a = kw.run(f, record=("p", "u_final"))
+    monkeypatch.setenv("KW_PLANE", "0")
+    assert not _uses_plane(kw, f)
+    b = kw.run(f, record=("p", "u_final"))
+    for k in a:
+        assert cases.rel_l2(a[k], b[k]) < 2e-6, k
