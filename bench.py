@@ -15,7 +15,8 @@ request stream run by the plain path (one simulation at a time) and by the
 serving loop (continuous batching), with throughput and p50/p99 latency for
 burst and Poisson arrivals.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload c2|c3|c4]
+(--workload c3/c4 times the large configurations, without the serving leg or the CPU baseline)
 """
 
 from __future__ import annotations
@@ -59,7 +60,7 @@ def kernel_bytes(f: dict, kind: str, launches_per_step: dict) -> float:
     if kind == "x_pressure":    # 2 G in, sum(rho), c0, tau, eta, Pk out
         fields = 1 + (1 if het("c0") else 0) + (2 if (het("c0") or het("alpha_coeff")) else 0)
         return 2 * spec + fields * fld + spec
-    if kind == "last_axis":     # averaged over the step's launches: in + out spectra + tables
+    if kind in ("last_axis", "yz_plane"):  # averaged over the step's launches: in + out spectra + tables
         # gradient 1->dim (+kappa), divergence dim->dim (+kappa), absorption 2->2 (+nabla1, nabla2)
         launches = [(1, dim, 1), (dim, dim, 1)] + ([(2, 2, 2)] if absorbing == 1 else [])
         tot = sum((i + o) * spec + t * tab for i, o, t in launches)
@@ -143,8 +144,10 @@ def serving_leg(kw, M: int, rates, max_active: int, quantum: int) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1200)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2",
+                    help="c2 (default): the 128^3 headline workload; c3/c4: the large configurations")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 1200 for c2, 100 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None, help="warmup steps (default: 100 for c2, 10 otherwise)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--calib-copy-mib", type=int, default=1024,
@@ -171,39 +174,67 @@ def main():
         print(f"calibration copy: {args.calib_copy_mib} MiB in {ms:.3f} ms ({copy_gbs:.0f} GB/s read+write)",
               file=sys.stderr, flush=True)
 
-    K, W = args.steps, args.warmup
-    f = configs.c2(Nt=max(K, W, 1))
+    wl = args.workload
+    K = args.steps if args.steps is not None else (1200 if wl == "c2" else 100)
+    W = args.warmup if args.warmup is not None else (100 if wl == "c2" else 10)
+    prof_steps = min(K, 200 if wl == "c2" else 50)
+    t_gen = time.perf_counter()
+    if wl == "c2":
+        f, rec, desc, bpp = configs.c2(Nt=max(K, W, 1)), configs.C2_RECORD, configs.C2_DESCRIPTION, 212.0
+        data = "synthetic (seeded Gaussian random medium, default_rng(1002))"
+    elif wl == "c3":
+        f, rec, desc, bpp = (configs.c3(Nt=W + K + prof_steps), configs.C3_RECORD, configs.C3_DESCRIPTION,
+                             208.0)
+        data = "synthetic (homogeneous medium, makeBowl source)"
+    else:
+        f, rec, desc, bpp = (configs.c4(Nt=W + K + prof_steps, log=True), configs.C4_RECORD,
+                             configs.C4_DESCRIPTION, 212.0)
+        data = "synthetic (seeded three-layer slab with Gaussian perturbation, default_rng(1004))"
+    print(f"{wl}: inputs generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
     Nx, Ny, Nz = (int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz"))
     N = Nx * Ny * Nz
-    rec = configs.C2_RECORD
 
-    # warmup on a separate instance
-    if W > 0:
-        with kw.Simulation(f, record=rec) as sim:
-            sim.run(W)
-            sim.sync()
-
-    # timed region: K steps of a fresh simulation from t = 1
-    with kw.Simulation(f, record=rec) as sim:
+    def timed(sim, steps):
         sim.sync()
         t0 = time.perf_counter()
-        sim.run(K)
+        sim.run(steps)
         sim.sync()
-        t1 = time.perf_counter()
-        p_max = sim.fetch("p_max")
-        if not np.all(np.isfinite(p_max)):
-            raise RuntimeError("non-finite p_max in the benchmark run")
-    elapsed = t1 - t0
+        return time.perf_counter() - t0
+
+    if wl == "c2":
+        # warmup on a separate instance; timed region: K steps of a fresh simulation from t = 1
+        if W > 0:
+            with kw.Simulation(f, record=rec) as sim:
+                sim.run(W)
+                sim.sync()
+        with kw.Simulation(f, record=rec) as sim:
+            elapsed = timed(sim, K)
+            chk = sim.fetch(rec[-1])
+        # per-kernel timing (HIP events around every launch, on the solver stream)
+        with kw.Simulation(f, record=rec) as sim:
+            sim.set_profiling(True)
+            sim.run(prof_steps)
+            sim.sync()
+            prof = sim.profile()
+    else:
+        # one resident simulation (tens of GB): W warmup steps, K timed, then profiled steps
+        with kw.Simulation(f, record=rec) as sim:
+            print(f"{wl}: simulation created ({kw.native.footprint_bytes(f, rec) / 2**30:.1f} GiB)",
+                  file=sys.stderr, flush=True)
+            if W > 0:
+                timed(sim, W)
+            elapsed = timed(sim, K)
+            print(f"{wl}: {K} timed steps in {elapsed:.2f} s", file=sys.stderr, flush=True)
+            sim.set_profiling(True)
+            sim.run(prof_steps)
+            sim.sync()
+            prof = sim.profile()
+            chk = sim.fetch(rec[-1])
+    if not np.all(np.isfinite(chk)):
+        raise RuntimeError(f"non-finite {rec[-1]} in the benchmark run")
+    del chk
     value = N * K / elapsed
     ms_per_step = 1e3 * elapsed / K
-
-    # per-kernel timing (HIP events around every launch, on the solver stream)
-    prof_steps = min(K, 200)
-    with kw.Simulation(f, record=rec) as sim:
-        sim.set_profiling(True)
-        sim.run(prof_steps)
-        sim.sync()
-        prof = sim.profile()
     kinds = {k: v for k, v in prof.items() if v[1] > 0}
     dom = max(kinds, key=lambda k: kinds[k][0])
     dom_ms, dom_n = kinds[dom]
@@ -213,7 +244,7 @@ def main():
     per_kernel = {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1],
                       "share": round(v[0] / sum(x[0] for x in kinds.values()), 3)} for k, v in kinds.items()}
 
-    step_bytes = 212.0 * N  # algorithmic bytes per time step of this workload (212 B/point)
+    step_bytes = bpp * N  # algorithmic bytes per time step of this workload
     out = {
         "metric": "grid-point-updates/s",
         "value": value,
@@ -226,25 +257,25 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Gaussian random medium, default_rng(1002))",
-        "config": {"workload": "C2 " + configs.C2_DESCRIPTION, "grid": [Nx, Ny, Nz], "Nt": K,
+        "data": data,
+        "config": {"workload": wl.upper() + " " + desc, "grid": [Nx, Ny, Nz], "Nt": K,
                    "record": list(rec)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": dom_bytes, "avg_launch_us": round(dom_avg_s * 1e6, 2)},
-        "step_roofline": {"bytes_per_point": 212, "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
+        "step_roofline": {"bytes_per_point": bpp, "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
                           "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": per_kernel,
     }
     if copy_gbs:
         out["copy_GBs"] = round(copy_gbs, 1)
 
-    if args.serving_requests > 0:
+    if args.serving_requests > 0 and wl == "c2":
         rates = [float(x) for x in args.serving_rates.split(",") if x]
         out["serving"] = serving_leg(kw, args.serving_requests, rates, args.serving_max_active,
                                      args.serving_quantum)
 
-    if not args.no_cpu:
+    if not args.no_cpu and wl == "c2":
         from oracle import kwave_oracle  # CPU baseline only
 
         cs = args.cpu_steps

@@ -11,6 +11,8 @@
 //   v[r] = x[i + r N/R] * W_N^(sign r k N/(NS R)),  k = i % NS
 //   y[(i/NS) NS R + k + r NS] = DFT_R(v)[r]
 #pragma once
+#include <type_traits>
+
 #include "kw_fft.hpp"
 
 namespace kwt {
@@ -41,6 +43,33 @@ constexpr int ns_of(int n, int s) {
     n /= r;
   }
   return p;
+}
+
+// The same radices in reverse order (a valid Stockham plan too).  Stage 0 of
+// the reversed plan reads, per work item, exactly the positions stage S-1 of
+// the forward plan writes (both have N / R_{S-1} butterflies of stride
+// N / R_{S-1}), so a forward transform's outputs can feed an inverse
+// transform from registers.
+constexpr int radix_rev(int n, int s) { return radix_of(n, num_stages(n) - 1 - s); }
+constexpr int ns_rev(int n, int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) p *= radix_rev(n, i);
+  return p;
+}
+template <bool REV>
+constexpr int plan_radix(int n, int s) { return REV ? radix_rev(n, s) : radix_of(n, s); }
+template <bool REV>
+constexpr int plan_ns(int n, int s) { return REV ? ns_rev(n, s) : ns_of(n, s); }
+
+// Loaders take (t, a) or (t, a, slot), slot = it * R + r as for stores.
+template <typename Load>
+__device__ __forceinline__ float2 call_load(Load& ld, int t, int a, int slot) {
+  if constexpr (std::is_invocable_v<Load&, int, int, int>) {
+    return ld(t, a, slot);
+  } else {
+    (void)slot;
+    return ld(t, a);
+  }
 }
 
 // radix-16 as 4 x 4 with internal twiddles W16^(n2 k1)
@@ -183,10 +212,10 @@ __device__ __forceinline__ void run(const float2* tw, float sign, int cnt, Load&
 // work item's inputs are loaded into registers, then a barrier, then the
 // butterflies and stores (a Stockham stage writes other positions than it
 // reads, so without the barrier a store could overwrite an unread input).
-template <int N, int S, int CMAX, int NT, bool TFAST, typename Load, typename Store>
+template <int N, int S, int CMAX, int NT, bool TFAST, bool REV = false, typename Load, typename Store>
 __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld, Store& st) {
-  constexpr int R = radix_of(N, S);
-  constexpr int NS = ns_of(N, S);
+  constexpr int R = plan_radix<REV>(N, S);
+  constexpr int NS = plan_ns<REV>(N, S);
   constexpr int NB = N / R;
   constexpr int TMAX = NB * CMAX;
   constexpr int TWSTEP = N / (NS * R);
@@ -199,7 +228,7 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
       const int t = TFAST ? w % CMAX : w / NB;
       const int i = TFAST ? w / CMAX : w % NB;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = ld(t, i + r * NB);
+      for (int r = 0; r < R; ++r) v[it][r] = call_load(ld, t, i + r * NB, it * R + r);
     }
   }
   __syncthreads();
@@ -230,27 +259,36 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
 // stage 0 loads through `first`, the last stage stores through `last`, the
 // others read and write L.  Stages are separated by barriers; there is no
 // trailing barrier.
-template <int N, int CMAX, int NT, bool TFAST, int S = 0, typename Load, typename Store, typename Lds>
+template <int N, int CMAX, int NT, bool TFAST, bool REV = false, int S = 0, typename Load, typename Store,
+          typename Lds>
 __device__ __forceinline__ void run_ip(const float2* tw, float sign, Load& first, Store& last, Lds L) {
   constexpr int NSTAGES = num_stages(N);
   static_assert(NSTAGES > 0, "length must factor into 2, 3, 5, 7");
   if constexpr (NSTAGES == 1) {
-    stage_ip<N, 0, CMAX, NT, TFAST>(tw, sign, first, last);
+    stage_ip<N, 0, CMAX, NT, TFAST, REV>(tw, sign, first, last);
   } else if constexpr (S == 0) {
     auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
-    stage_ip<N, 0, CMAX, NT, TFAST>(tw, sign, first, st);
+    stage_ip<N, 0, CMAX, NT, TFAST, REV>(tw, sign, first, st);
     __syncthreads();
-    run_ip<N, CMAX, NT, TFAST, 1>(tw, sign, first, last, L);
+    run_ip<N, CMAX, NT, TFAST, REV, 1>(tw, sign, first, last, L);
   } else if constexpr (S == NSTAGES - 1) {
     auto ld = [&](int t, int a) { return L(t, a); };
-    stage_ip<N, S, CMAX, NT, TFAST>(tw, sign, ld, last);
+    stage_ip<N, S, CMAX, NT, TFAST, REV>(tw, sign, ld, last);
   } else {
     auto ld = [&](int t, int a) { return L(t, a); };
     auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
-    stage_ip<N, S, CMAX, NT, TFAST>(tw, sign, ld, st);
+    stage_ip<N, S, CMAX, NT, TFAST, REV>(tw, sign, ld, st);
     __syncthreads();
-    run_ip<N, CMAX, NT, TFAST, S + 1>(tw, sign, first, last, L);
+    run_ip<N, CMAX, NT, TFAST, REV, S + 1>(tw, sign, first, last, L);
   }
+}
+
+// Registers a thread holds between a forward run_ip (last stage stores by
+// slot) and the reversed-plan inverse (first stage loads by slot).
+template <int N, int CMAX, int NT>
+constexpr int handoff_slots() {
+  constexpr int R = radix_of(N, num_stages(N) - 1);
+  return ((N / R) * CMAX + NT - 1) / NT * R;
 }
 
 // Number of output slots a thread owns in the last stage, and the (t, a)

@@ -92,16 +92,19 @@ def _signal(sig: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(s.T[None].astype(np.float32))
 
 
-def _stagger(rho0: np.ndarray, axis: int) -> np.ndarray:
-    """interpn('*linear') at +d/2 along axis, last plane copied (3D.m:583-596)."""
-    r = np.asarray(rho0, dtype=np.float64)
-    out = r.copy()
-    sl_a = [slice(None)] * r.ndim
-    sl_b = [slice(None)] * r.ndim
-    sl_a[axis] = slice(0, -1)
-    sl_b[axis] = slice(1, None)
-    out[tuple(sl_a)] = 0.5 * (r[tuple(sl_a)] + r[tuple(sl_b)])
-    return out
+def _stagger_c(rc: np.ndarray, axis: int) -> np.ndarray:
+    """interpn('*linear') at +d/2 along a MATLAB axis (0 = x), last plane copied
+    (3D.m:583-596), on a C-order (Nz, Ny, Nx) float64 array; returns float32."""
+    ca = 2 - axis
+    n = rc.shape[ca]
+    a = [slice(None)] * 3
+    b = [slice(None)] * 3
+    a[ca] = slice(0, n - 1)
+    b[ca] = slice(1, n)
+    out = rc.copy()
+    out[tuple(a)] += rc[tuple(b)]
+    out[tuple(a)] *= 0.5
+    return out.astype(np.float32)
 
 
 def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict,
@@ -142,7 +145,8 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
         c_ref = float(sound_speed_ref)
     dt = kgrid.dt
     d = [kgrid.dx, kgrid.dy, kgrid.dz]
-    c_at = (lambda i: float(c0.reshape(-1, order="F")[i - 1])) if hetero_c else (lambda i: c0_s)
+    c0_flat = c0.reshape(-1, order="F") if hetero_c else None   # one reshape (a copy unless F-ordered)
+    c_at = (lambda i: float(c0_flat[i - 1])) if hetero_c else (lambda i: c0_s)
 
     f = {}
     f["Nx"], f["Ny"], f["Nz"], f["Nt"] = (_scalar(kgrid.Nx, np.uint64), _scalar(kgrid.Ny, np.uint64),
@@ -158,12 +162,14 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
     f["dt"] = _scalar(dt)
     f["c0"] = _c_order(c0).astype(np.float32) if hetero_c else _scalar(c0_s)
     f["c_ref"] = _scalar(c_ref)
-    f["rho0"] = _c_order(rho0).astype(np.float32) if hetero_rho else _scalar(float(rho0.reshape(-1)[0]))
+    rho_c = _c_order(rho0) if hetero_rho else None
+    f["rho0"] = rho_c.astype(np.float32) if hetero_rho else _scalar(float(rho0.reshape(-1)[0]))
     for a, nm in enumerate("xyz"[:dim]):
         if hetero_rho:
-            f["rho0_sg" + nm] = _c_order(_stagger(rho0, a)).astype(np.float32)
+            f["rho0_sg" + nm] = _stagger_c(rho_c, a)
         else:
             f["rho0_sg" + nm] = _scalar(float(rho0.reshape(-1)[0]))
+    del rho_c
 
     # flags (writeFlags.m, saveToDisk.m:88-106)
     zero = _scalar(0, np.uint64)
@@ -249,3 +255,55 @@ def plane_wave_check_stable(kgrid: KWaveGrid, c_max: float) -> float:
     inv = (1 / kgrid.dx ** 2 + 1 / kgrid.dy ** 2 + (1 / kgrid.dz ** 2 if kgrid.dim == 3 else 0))
     kmax = math.pi * math.sqrt(inv)
     return 2.0 / (c_max * kmax)
+
+
+class CartesianSensor:
+    """Cartesian sensor points with 'linear' interpolation, for the binary-mask contract.
+
+    kspaceFirstOrder2D interpolates p at Cartesian sensor points from the
+    Delaunay triangulation of the grid points and the barycentric coordinates
+    of each point in its triangle (kspaceFirstOrder_createStorageVariables.m:
+    370-397 -> gridDataFast2D.m; applied each step as sum(p(tri) .* bc, 2),
+    kspaceFirstOrder_extractSensorData.m:297-306).  The external binaries take
+    binary masks only (kspaceFirstOrder_saveToDisk.m:35-37), so here the mask
+    is the union of the triangle vertices and the weights are applied to the
+    recorded vertex time series afterwards.
+
+    Triangulation rule: the Delaunay triangulation of a regular grid is
+    degenerate (every cell is cocircular), so which diagonal splits a cell is
+    implementation-defined; this rule always splits cell (i, j) along the
+    diagonal from grid point (i, j) to (i+1, j+1).  Interpolation of any field
+    linear in x and y is exact under either choice.
+    """
+
+    def __init__(self, kgrid: KWaveGrid, cart_pos):
+        if kgrid.dim != 2:
+            raise NotImplementedError("Cartesian sensors are implemented for 2-D grids")
+        pos = np.asarray(cart_pos, dtype=np.float64).reshape(2, -1)
+        Nx, Ny = kgrid.Nx, kgrid.Ny
+        # kgrid.x_vec(i) = (i - floor(N/2)) * dx, 0-based i (kWaveGrid.m:494-495, makeDim 779-797)
+        gx = pos[0] / kgrid.dx + Nx // 2
+        gy = pos[1] / kgrid.dy + Ny // 2
+        if np.any(gx < 0) or np.any(gx > Nx - 1) or np.any(gy < 0) or np.any(gy > Ny - 1):
+            raise ValueError("Cartesian points must lie within the k-space grid defined by kgrid.")
+        i = np.minimum(np.floor(gx).astype(np.int64), Nx - 2)
+        j = np.minimum(np.floor(gy).astype(np.int64), Ny - 2)
+        ux, uy = gx - i, gy - j
+        lower = ux >= uy
+        v1x = np.where(lower, i + 1, i)
+        v1y = np.where(lower, j, j + 1)
+        vx = np.stack([i, v1x, i + 1], axis=1)
+        vy = np.stack([j, v1y, j + 1], axis=1)
+        self.bc = np.stack([1.0 - np.maximum(ux, uy), np.abs(ux - uy), np.minimum(ux, uy)], axis=1)
+        lin = vx + Nx * vy                                  # 0-based column-major index
+        uniq = np.unique(lin)                               # find() order of the mask
+        flat = np.zeros(Nx * Ny)
+        flat[uniq] = 1.0
+        self.mask = flat.reshape((Nx, Ny), order="F")
+        self.tri = np.searchsorted(uniq, lin)               # column of each vertex in the recorded data
+        self.num_points = pos.shape[1]
+
+    def apply(self, rec: np.ndarray) -> np.ndarray:
+        """Interpolate recorded mask data (..., Nmask) -> (..., Ns) in float64."""
+        r = np.asarray(rec, dtype=np.float64)
+        return np.einsum("...sk,sk->...s", r[..., self.tri], self.bc)

@@ -52,6 +52,77 @@ def make_ball(Nx, Ny, Nz, cx, cy, cz, radius) -> np.ndarray:
                    axis=(0, 1, 2))
 
 
+def _mround(x):
+    """MATLAB round (halves away from zero)."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+def make_bowl(grid_size, bowl_pos, radius, diameter, focus_pos) -> np.ndarray:
+    """Binary bowl (spherical cap) in a 3-D grid.  makeBowl.m:80-710 (default options).
+
+    grid_size (Nx, Ny, Nz); bowl_pos (centre of the rear surface) and
+    focus_pos (any point on the axis) are 1-based grid points; radius of
+    curvature and aperture diameter (odd) in grid points.  The surface is
+    found in a bounding box of side round(sqrt(2) d) + 2 by a forward and a
+    backward line search along x, then y, then z for the point whose distance
+    from the sphere centre is closest to the radius (kept if within 0.5),
+    then trimmed to the cap's half-angle asin(d / 2R) about the axis.
+    """
+    grid = [int(_mround(v)) for v in grid_size]
+    bpos = [int(_mround(v)) for v in bowl_pos]
+    fpos = [int(_mround(v)) for v in focus_pos]
+    diameter = int(_mround(diameter))
+    radius = float(_mround(radius))
+    if diameter > 2 * radius or diameter % 2 == 0 or bpos == fpos:
+        raise ValueError("invalid bowl: diameter must be odd and <= 2 radius, focus_pos != bowl_pos")
+    n = int(_mround(math.sqrt(2) * diameter)) + 2
+    b = [math.ceil(n / 2)] * 3                                    # bowl_pos in the box (1-based)
+    f = [b[i] + (fpos[i] - bpos[i]) for i in range(3)]
+    half_arc = math.asin(diameter / (2 * radius))
+    dist = math.sqrt(sum((b[i] - f[i]) ** 2 for i in range(3)))
+    c = [float(_mround(radius / dist * (f[i] - b[i]) + b[i])) for i in range(3)]
+    ax = [np.arange(1, n + 1, dtype=np.float64) - c[i] for i in range(3)]
+    pm = np.abs(np.sqrt(ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2)
+                - radius)
+    bowl = np.zeros((n, n, n), dtype=bool)
+    for axis in range(3):
+        if axis > 0:
+            pm[bowl] = 0.0
+        m = np.moveaxis(pm, axis, 0)
+        for backward in (False, True):
+            mm = m[::-1] if backward else m
+            idx = np.argmin(mm, axis=0)                            # first minimum, as min(..., [], 1)
+            val = np.take_along_axis(mm, idx[None], axis=0)[0]
+            sel = np.nonzero(val < 0.5)
+            pos = idx[sel]
+            if backward:
+                pos = n - 1 - pos
+            coord = [None, None, None]
+            others = [a for a in range(3) if a != axis]
+            coord[axis] = pos
+            coord[others[0]], coord[others[1]] = sel
+            bowl[tuple(coord)] = True
+    # trim to the cap
+    pts = np.argwhere(bowl).astype(np.float64) + 1.0
+    v1 = np.array(b, dtype=np.float64) - np.array(c)
+    v2 = pts - np.array(c)
+    cosang = (v2 @ v1) / (np.linalg.norm(v1) * np.linalg.norm(v2, axis=1))
+    theta = np.arccos(np.clip(cosang, -1.0, 1.0))
+    drop = pts[theta > half_arc].astype(np.int64) - 1
+    bowl[tuple(drop.T)] = False
+    # place the box in the grid (cropping what falls outside)
+    out = np.zeros(grid, dtype=np.float64)
+    lo = [bpos[i] - b[i] for i in range(3)]                       # 0-based grid index of box index 0
+    src, dst = [], []
+    for i in range(3):
+        a0, a1 = max(0, -lo[i]), min(n, grid[i] - lo[i])
+        src.append(slice(a0, a1))
+        dst.append(slice(lo[i] + a0, lo[i] + a1))
+    out[tuple(dst)] = bowl[tuple(src)]
+    return out
+
+
 def make_cart_circle(radius: float, num_points: int, center=(0.0, 0.0),
                      arc_angle: float = 2 * math.pi) -> np.ndarray:
     """Cartesian points on a circle, shape (2, num_points).  makeCartCircle.m:54-97."""

@@ -154,3 +154,27 @@ def test_plane_pass_matches_three_pass(kw, monkeypatch):
     b = kw.run(f, record=("p", "u_final"))
     for k in a:
         assert cases.rel_l2(a[k], b[k]) < 2e-6, k
+
+
+def test_c1_cartesian_sensor(kw, oracle):
+    """C1 at full size (128^2, Nt 500): GPU vertex series -> Cartesian points vs the oracle."""
+    f, s = kw.configs.c1()
+    print("[gpu parity] C1 128^2 Nt 500 Cartesian sensor", flush=True)
+    g = s.apply(kw.run(f, record=("p",))["p"])
+    r = s.apply(oracle.run(f, record=("p",))["p"])
+    assert g.shape == (500, 50)
+    e = cases.rel_l2(g, r)
+    print(f"    p (50 Cartesian points): rel L2 {e:.2e}", flush=True)
+    assert e <= TOL_L2
+
+
+def test_c3_reduced(kw, oracle):
+    """C3's branch (nonlinear, y=2 absorbing, bowl p source, all-point p_max/p_rms) on 64x64x128."""
+    f = kw.configs.c3(Nt=100, shape=(64, 64, 128), radius=24, diameter=31, pml=6)
+    compare(kw, oracle, f, ("p_max", "p_rms"), "c3_reduced")
+
+
+def test_c4_reduced(kw, oracle):
+    """C4's branch (three-layer slab, y=1.1, p additive patch, receiver plane) on 96^3."""
+    f = kw.configs.c4(Nt=60, N=96)
+    compare(kw, oracle, f, ("p",), "c4_reduced")
