@@ -654,7 +654,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     s->tll = choose_tile(g.dim == 3 ? g.Nxh * g.Ny : g.Nxh, s->axL.n, 3);
     // workgroup size of the length-specialised kernels (0: generic kernels)
     const int pencil_nt = env_nt("KW_PENCIL_NT", 256);
-    int row_nt = env_nt("KW_ROW_NT", 128);
+    int row_nt = env_nt("KW_ROW_NT", g.Nx >= 256 ? 256 : 128);
     s->xl.nt = fast_length(g.Nx) ? row_nt : 0;
     s->tly.nt = (g.dim == 3 && fast_length(g.Ny)) ? pencil_nt : 0;
     s->tll.nt = fast_length(s->axL.n) ? pencil_nt : 0;
@@ -674,7 +674,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
       const bool allow = !(pe && pe[0] == '0');
       g.kxm = (allow && need > 0 && need <= (size_t)lds_max && s->xl.nt > 0) ? 1 : 0;
       // kx-major rows stage their spectra through LDS: 256-thread row kernels
-      // (measured fastest there; 128 otherwise)
+      // (measured fastest there; otherwise 256 for Nx >= 256 and 128 below)
       if (g.kxm && !std::getenv("KW_ROW_NT")) s->xl.nt = 256;
     }
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
