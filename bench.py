@@ -71,6 +71,22 @@ def kernel_bytes(f: dict, kind: str, launches_per_step: dict) -> float:
     return 0.0
 
 
+def profiled_traffic(workload: str, kind: str):
+    """HBM bytes per launch of a kernel kind at this workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/<workload>_*.md, calibrated
+    against a plain copy); None when no summary covers it."""
+    import glob
+    import re
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"{workload}_*.md"))):
+        m = re.search(r"```json\s*(\{.*?\})\s*```", open(path).read(), re.S)
+        if m:
+            t = json.loads(m.group(1)).get("traffic_bytes_per_launch", {})
+            if kind in t:
+                return float(t[kind]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def _updates(f: dict) -> int:
     return int(np.prod([int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz", "Nt")]))
 
@@ -244,6 +260,7 @@ def main():
     per_kernel = {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1],
                       "share": round(v[0] / sum(x[0] for x in kinds.values()), 3)} for k, v in kinds.items()}
 
+    traffic, traffic_src = profiled_traffic(wl, dom)
     step_bytes = bpp * N  # algorithmic bytes per time step of this workload
     out = {
         "metric": "grid-point-updates/s",
@@ -261,8 +278,9 @@ def main():
         "config": {"workload": wl.upper() + " " + desc, "grid": [Nx, Ny, Nz], "Nt": K,
                    "record": list(rec)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": dom_bytes, "avg_launch_us": round(dom_avg_s * 1e6, 2)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src, "bytes_per_launch": dom_bytes,
+                     "avg_launch_us": round(dom_avg_s * 1e6, 2)},
         "step_roofline": {"bytes_per_point": bpp, "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
                           "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": per_kernel,

@@ -77,3 +77,54 @@ def test_mirror_symmetry_and_determinism(kw):
     sym = cases.rel_l2(a, a[:, ::-1, :])
     print(f"    mirror asymmetry {sym:.2e}", flush=True)
     assert sym < 1e-4, sym
+
+
+def test_c3_full_size_window_against_oracle(kw, oracle):
+    """C3 at its full size (256x256x512, nonlinear, y=2 absorbing, bowl source, every
+    point recording p_max/p_rms): the first 12 steps against the FP64 oracle."""
+    from kwave_amd import configs
+
+    print("[gpu full] C3 256x256x512, 12 steps vs oracle (all-point p_max, p_rms)", flush=True)
+    f = configs.c3(Nt=12)
+    rec = ("p_max", "p_rms")
+    gpu = kw.run(f, record=rec)
+    ref = oracle.run(f, record=rec)
+    for k in rec:
+        e = cases.rel_l2(gpu[k], ref[k])
+        print(f"    {k}: rel L2 {e:.2e}", flush=True)
+        assert e < 1e-4, (k, e)
+
+
+def test_c3_full_size_properties(kw):
+    """C3 over 400 steps: finite, p_min <= p_max, p_rms <= max|p|, and the field's
+    maximum lies under the bowl aperture."""
+    from kwave_amd import configs
+
+    print("[gpu full] C3 256x256x512, 400 steps, sensor-statistics properties", flush=True)
+    f = configs.c3(Nt=400)
+    out = kw.run(f, record=("p_max", "p_min", "p_rms"))
+    pmax, pmin, prms = out["p_max"], out["p_min"], out["p_rms"]
+    assert np.all(np.isfinite(pmax)) and np.all(np.isfinite(pmin)) and np.all(np.isfinite(prms))
+    assert np.all(pmin <= pmax)
+    assert np.all(prms <= np.maximum(np.abs(pmax), np.abs(pmin)) * (1 + 1e-5) + 1e-3)
+    j = int(np.argmax(pmax))                     # MATLAB linear index order (x fastest)
+    x, y = j % 256, (j // 256) % 256
+    assert abs(x - 127) <= 64 and abs(y - 127) <= 64, (x, y)
+
+
+def test_c4_full_size_linearity(kw):
+    """C4 at its full size (768^3, linear absorbing slab): doubling the source signal
+    doubles the field (scaling by 2 commutes with rounding except where values fall
+    into the subnormal range, hence a 1e-6 bound rather than bit equality)."""
+    from kwave_amd import configs
+
+    print("[gpu full] C4 768^3, 30 steps, source linearity", flush=True)
+    f = configs.c4(Nt=30)
+    a = kw.run(f, record=("p_final",))["p_final"]
+    f2 = dict(f)
+    f2["p_source_input"] = (2 * np.asarray(f["p_source_input"], dtype=np.float32)).astype(np.float32)
+    b = kw.run(f2, record=("p_final",))["p_final"]
+    assert np.all(np.isfinite(a)) and np.abs(a).max() > 0
+    err = np.abs(b - 2 * a).max() / np.abs(2 * a).max()
+    print(f"    max |p(2s) - 2 p(s)| / max |2 p(s)| = {err:.2e}", flush=True)
+    assert err < 1e-6
