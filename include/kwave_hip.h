@@ -63,7 +63,10 @@ enum kw_record_flag {
   KW_REC_U_RMS = 1u << 10,
   KW_REC_U_MAX_ALL = 1u << 11,
   KW_REC_U_MIN_ALL = 1u << 12,
-  KW_REC_U_FINAL = 1u << 13
+  KW_REC_U_FINAL = 1u << 13,
+  /* --u_non_staggered_raw: u shifted to the unstaggered grid points,
+   * real(ifft(x_shift_neg .* fft(u, [], axis))) (extractSensorData.m:39-55) */
+  KW_REC_U_NON_STAGGERED = 1u << 14
 };
 
 /* Output datasets (names of the output file read at kspaceFirstOrder3DC.m:568-778).
@@ -83,7 +86,8 @@ typedef enum kw_output_id {
   KW_OUT_UX_MAX_ALL = 19,
   KW_OUT_UX_MIN_ALL = 22,
   KW_OUT_UX_FINAL = 25,
-  KW_OUT_COUNT = 28
+  KW_OUT_UX_NON_STAGGERED = 28, /* +axis: (Nt_rec, Ns) */
+  KW_OUT_COUNT = 31
 } kw_output_id;
 
 /* One simulation: the dataset set of the HDF5 input file

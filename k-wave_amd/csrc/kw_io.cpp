@@ -545,7 +545,7 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
     int kind;
   } uspecs[] = {{KW_OUT_UX, "", 0},          {KW_OUT_UX_MAX, "_max", 1},         {KW_OUT_UX_MIN, "_min", 1},
                 {KW_OUT_UX_RMS, "_rms", 1},  {KW_OUT_UX_MAX_ALL, "_max_all", 2}, {KW_OUT_UX_MIN_ALL, "_min_all", 2},
-                {KW_OUT_UX_FINAL, "_final", 2}};
+                {KW_OUT_UX_FINAL, "_final", 2},  {KW_OUT_UX_NON_STAGGERED, "_non_staggered", 0}};
   for (const auto& u : uspecs)
     for (int a = 0; a < dim; ++a) emit(u.base + a, std::string("u") + axes[a] + u.suffix, u.kind);
   if (!st) st = kwio_write_root_attributes(out, "output", "Output data created by kspaceFirstOrder-HIP");

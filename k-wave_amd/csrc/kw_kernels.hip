@@ -584,6 +584,10 @@ __global__ void __launch_bounds__(256) k_last(Grid g, FftAxis ax, const float2* 
             const int kk = so.axis == 0 ? kx : (so.axis == 1 ? ky : kz);
             v = cscale(cmul(dv[kk], v), T.kappa[sidx]);
           } break;
+          case OP_SHIFT: {
+            const int kk = so.axis == 0 ? kx : (so.axis == 1 ? ky : kz);
+            v = cmul(T.shift[so.axis][kk], v);
+          } break;
           case OP_NABLA1: v = cscale(v, T.nabla1[sidx]); break;
           case OP_NABLA2: v = cscale(v, T.nabla2[sidx]); break;
           case OP_SKAPPA: v = cscale(v, T.skappa[sidx]); break;
@@ -625,7 +629,8 @@ __device__ __forceinline__ double kvec(int j, int n, double d) {
 }
 
 // ddx_k_shift_pos/neg = i k exp(+-i k d/2)  (3D.m:627-634)
-__global__ void k_dvec(float2* dpos, float2* dneg, int nfull, int nout, double d) {
+// shift: x_shift_neg = exp(-i k d/2) (kspaceFirstOrder_createStorageVariables.m:87-99)
+__global__ void k_dvec(float2* dpos, float2* dneg, float2* shift, int nfull, int nout, double d) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nout) return;
   const double k = kvec(j, nfull, d);
@@ -633,6 +638,15 @@ __global__ void k_dvec(float2* dpos, float2* dneg, int nfull, int nout, double d
   sincos(k * d / 2.0, &s, &c);
   dpos[j] = make_float2((float)(-k * s), (float)(k * c));
   dneg[j] = make_float2((float)(k * s), (float)(k * c));
+  // real(ifft(...)) drops the imaginary Nyquist term i*U_N of a real u, so the
+  // shift is 0 there (keeps the product Hermitian for the half-spectrum path)
+  if (shift) shift[j] = (nfull % 2 == 0 && 2 * j == nfull) ? make_float2(0.f, 0.f) : make_float2((float)c, (float)(-s));
+}
+
+__global__ void k_gather(const float* __restrict__ field, const long long* __restrict__ lin, int n,
+                         float* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = field[lin[j]];
 }
 
 // getPML.m:50-85
@@ -831,17 +845,30 @@ void launch_x_source(hipStream_t s, const Grid& g, const FftAxis& ax, const floa
 
 void launch_x_c2r(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const float2* in, float* out,
                   XLaunch xl) {
-  if (KW_BY_NT(xl.nt, launch_x_c2r_fast_t, s, g, tw, in, out)) return;
+  if (KW_BY_NT(xl.nt, launch_x_c2r_fast_t, s, g, tw, in, out, (const float2*)nullptr)) return;
   hipLaunchKernelGGL(k_x_c2r, dim3(x_blocks(g, xl.pairs)), dim3(xl.threads), x_smem(g, xl.pairs, 2, false), s, g,
                      ax, tw, in, out, xl.pairs);
+}
+
+bool launch_x_c2r_mult(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out,
+                       const float2* mult, XLaunch xl) {
+  return KW_BY_NT(xl.nt, launch_x_c2r_fast_t, s, g, tw, in, out, mult);
+}
+
+bool launch_x_r2c(hipStream_t s, const Grid& g, const float2* tw, const float* in, float2* out, XLaunch xl) {
+  return KW_BY_NT(xl.nt, launch_x_r2c_fast_t, s, g, tw, in, out);
+}
+
+void launch_gather(hipStream_t s, const float* field, const long long* lin, int n, float* out) {
+  if (n > 0) hipLaunchKernelGGL(k_gather, dim3(nblk(n, 256)), dim3(256), 0, s, field, lin, n, out);
 }
 
 void launch_twiddles(hipStream_t s, float2* tw, int n) {
   hipLaunchKernelGGL(k_twiddles, dim3(nblk(n, 256)), dim3(256), 0, s, tw, n);
 }
 
-void launch_dvec(hipStream_t s, float2* dpos, float2* dneg, int nfull, int nout, double d) {
-  hipLaunchKernelGGL(k_dvec, dim3(nblk(nout, 256)), dim3(256), 0, s, dpos, dneg, nfull, nout, d);
+void launch_dvec(hipStream_t s, float2* dpos, float2* dneg, float2* shift, int nfull, int nout, double d) {
+  hipLaunchKernelGGL(k_dvec, dim3(nblk(nout, 256)), dim3(256), 0, s, dpos, dneg, shift, nfull, nout, d);
 }
 
 void launch_pml(hipStream_t s, float* pml, int n, double d, double dt, double c, int size, double alpha,

@@ -66,7 +66,9 @@ struct Sens {
   float* final_field;  // write the field (p_final) when non-null
 };
 
-enum SpecOp { OP_SCALE = 0, OP_DPOS_K = 1, OP_DNEG_K = 2, OP_NABLA1 = 3, OP_NABLA2 = 4, OP_SKAPPA = 5 };
+// OP_SHIFT: the half-cell shift exp(-i k d/2) along one axis, no kappa
+// (u_non_staggered, extractSensorData.m:39-55)
+enum SpecOp { OP_SCALE = 0, OP_DPOS_K = 1, OP_DNEG_K = 2, OP_NABLA1 = 3, OP_NABLA2 = 4, OP_SKAPPA = 5, OP_SHIFT = 6 };
 
 struct SpecOut {
   int src;   // input index
@@ -77,6 +79,7 @@ struct SpecOut {
 struct SpecTables {
   const float2* dpos[3];
   const float2* dneg[3];
+  const float2* shift[3];  // exp(-i k d/2) per axis (u_non_staggered only)
   const float* kappa;
   const float* skappa;
   const float* nabla1;
@@ -136,7 +139,10 @@ bool launch_x_init_fast_t(hipStream_t s, const Grid& g, const float2* tw, const 
 template <int NT>
 bool launch_x_source_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Src& src, int unit, float2* out);
 template <int NT>
-bool launch_x_c2r_fast_t(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out);
+bool launch_x_c2r_fast_t(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out,
+                         const float2* mult);
+template <int NT>
+bool launch_x_r2c_fast_t(hipStream_t s, const Grid& g, const float2* tw, const float* in, float2* out);
 
 void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, float2* const* f,
                   int nf, float sign, TileLaunch tl);
@@ -157,6 +163,13 @@ void launch_x_source(hipStream_t s, const Grid& g, const FftAxis& ax, const floa
                      int unit, float2* out, XLaunch xl);
 void launch_x_c2r(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const float2* in,
                   float* out, XLaunch xl);
+// x C2R of (mult[kx] .* in) into a real field, and x R2C of a real field
+// (specialised row lengths only; false when Nx has no compiled kernels)
+bool launch_x_c2r_mult(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out,
+                       const float2* mult, XLaunch xl);
+bool launch_x_r2c(hipStream_t s, const Grid& g, const float2* tw, const float* in, float2* out, XLaunch xl);
+// out[j] = field[lin[j]], j < n
+void launch_gather(hipStream_t s, const float* field, const long long* lin, int n, float* out);
 
 // FP64 precompute
 struct OpParams {
@@ -166,7 +179,7 @@ struct OpParams {
   int absorbing;
 };
 void launch_twiddles(hipStream_t s, float2* tw, int n);
-void launch_dvec(hipStream_t s, float2* dpos, float2* dneg, int nfull, int nout, double d);
+void launch_dvec(hipStream_t s, float2* dpos, float2* dneg, float2* shift, int nfull, int nout, double d);
 void launch_pml(hipStream_t s, float* pml, int n, double d, double dt, double c, int size, double alpha,
                 int staggered);
 void launch_spectral_tables(hipStream_t s, const Grid& g, const OpParams& p, float* kappa, float* skappa,

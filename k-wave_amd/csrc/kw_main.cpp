@@ -20,7 +20,7 @@ static void usage(FILE* out) {
           "  -c <level>          deflate level of the output datasets\n"
           "  --verbose <0..2>\n"
           "  --p_raw --p_max --p_min --p_rms --p_max_all --p_min_all --p_final\n"
-          "  --u_raw --u_max --u_min --u_rms --u_max_all --u_min_all --u_final\n"
+          "  --u_raw --u_max --u_min --u_rms --u_max_all --u_min_all --u_final --u_non_staggered_raw\n"
           "  (no output flag: --p_raw)\n");
 }
 
@@ -36,7 +36,8 @@ int main(int argc, char** argv) {
                {"--p_rms", KW_REC_P_RMS},     {"--p_max_all", KW_REC_P_MAX_ALL}, {"--p_min_all", KW_REC_P_MIN_ALL},
                {"--p_final", KW_REC_P_FINAL}, {"--u_raw", KW_REC_U},             {"--u_max", KW_REC_U_MAX},
                {"--u_min", KW_REC_U_MIN},     {"--u_rms", KW_REC_U_RMS},         {"--u_max_all", KW_REC_U_MAX_ALL},
-               {"--u_min_all", KW_REC_U_MIN_ALL}, {"--u_final", KW_REC_U_FINAL}};
+               {"--u_min_all", KW_REC_U_MIN_ALL}, {"--u_final", KW_REC_U_FINAL},
+               {"--u_non_staggered_raw", KW_REC_U_NON_STAGGERED}};
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto val = [&](const char* what) -> const char* {
@@ -73,7 +74,7 @@ int main(int argc, char** argv) {
     } else if (a == "--version") {
       printf("kspaceFirstOrder-HIP (gfx950), HDF5 file format 1.2\n");
       return 0;
-    } else if (a == "--u_non_staggered_raw" || a == "--copy_sensor_mask" || a == "--checkpoint_file" ||
+    } else if (a == "--copy_sensor_mask" || a == "--checkpoint_file" ||
                a == "--checkpoint_interval" || a == "--benchmark") {
       fprintf(stderr, "kspaceFirstOrder-HIP: option %s is not supported by this build\n", a.c_str());
       return 3;

@@ -133,6 +133,12 @@ struct kw_sim {
         *p_final = nullptr;
   double* p_rms = nullptr;
   float *u_raw[3] = {}, *u_max[3] = {}, *u_min[3] = {}, *u_max_all[3] = {}, *u_min_all[3] = {};
+  // u_non_staggered: per-axis series, a spectrum and a real field of scratch,
+  // and the x shift pre-divided by Nx (the x C2R is unnormalised)
+  float* u_ns[3] = {};
+  float2* ns_spec = nullptr;
+  float* ns_field = nullptr;
+  float2* ns_xmult = nullptr;
   double* u_rms[3] = {};
   long long nrec_total = 0;  // Nt - start + 1
   // profiling
@@ -392,6 +398,34 @@ Sens make_sens_u(const kw_sim* s, long long t, int a) {
 }
 
 // One time step t (1-based) of kspaceFirstOrder3D.m:767-1206.
+// u_non_staggered at the sensor points (extractSensorData.m:39-55): per axis,
+// real(ifft(shift_neg .* fft(u, [], axis), [], axis)).  x: C2R_x(shift_x / Nx .*
+// R2C_x(u)); y, z: the spectral pass with OP_SHIFT along the axis (the other
+// transforms cancel), then C2R_x; each gathered at the sensor points.
+void record_non_staggered(kw_sim* s, hipStream_t st, long long t) {
+  if (!s->u_ns[0] || t < s->start || s->ns == 0) return;
+  const Grid& g = s->g;
+  const long long row = t - s->start;
+  for (int a = 0; a < g.dim; ++a) {
+    if (!launch_x_r2c(st, g, s->twX, s->u[a], s->ns_spec, s->xl))
+      throw HipError{hipErrorInvalidConfiguration, "x R2C for u_non_staggered"};
+    const float2* mult = nullptr;
+    if (a == 0) {
+      mult = s->ns_xmult;
+    } else {
+      LastArgs la{};
+      la.in[0] = la.out[0] = s->ns_spec;
+      la.nin = la.nout = 1;
+      la.outs[0] = SpecOut{0, OP_SHIFT, a};
+      la.scale = (float)(1.0 / (double)g.N);
+      spectral(s, st, la);
+    }
+    if (!launch_x_c2r_mult(st, g, s->twX, s->ns_spec, s->ns_field, mult, s->xl))
+      throw HipError{hipErrorInvalidConfiguration, "x C2R for u_non_staggered"};
+    launch_gather(st, s->ns_field, s->s_lin, s->ns, s->u_ns[a] + (size_t)row * s->ns);
+  }
+}
+
 void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
   const Grid& g = s->g;
   const int dim = g.dim;
@@ -429,6 +463,7 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
       Prof pr(s, st, KK_XINIT);
       launch_x_init(st, g, s->axX, s->twX, s->m, s->p0, s->rho, s->Pk, none, s->xl);
     }
+    record_non_staggered(s, st, t);
     return;
   }
 
@@ -489,6 +524,7 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
     Prof pr(s, st, KK_XPRES);
     launch_x_pressure(st, g, s->axX, s->twX, s->m, s->G[0], s->G[1], s->srho, s->Pk, ps, s->xl);
   }
+  record_non_staggered(s, st, t);
 }
 
 kw_status setup_source(kw_sim* s, SrcData& d, uint64_t flag, uint64_t mode, uint64_t many, uint64_t npoints,
@@ -553,6 +589,7 @@ uint64_t kw_sim_footprint_bytes(const kw_sim_desc* d) {
   const uint64_t ns = d->sensor_npoints;
   if (d->record_flags & KW_REC_P) b += ns * nrec * 4;
   if (d->record_flags & KW_REC_U) b += dim * ns * nrec * 4;
+  if (d->record_flags & KW_REC_U_NON_STAGGERED) b += dim * ns * nrec * 4 + nspec * 8 + N * 4;
   if (d->record_flags & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL | KW_REC_P_FINAL)) b += 3 * N * 4;
   if (d->record_flags & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL)) b += 2 * dim * N * 4;
   b += 8 * ns * 16;
@@ -658,6 +695,11 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     s->xl.nt = fast_length(g.Nx) ? row_nt : 0;
     s->tly.nt = (g.dim == 3 && fast_length(g.Ny)) ? pencil_nt : 0;
     s->tll.nt = fast_length(s->axL.n) ? pencil_nt : 0;
+    if ((d->record_flags & KW_REC_U_NON_STAGGERED) && !s->xl.nt) {
+      delete s;
+      return fail(KW_ERR_UNSUPPORTED, "u_non_staggered needs Nx among the compiled row lengths "
+                                      "(32, 48, 64, 96, 128, 160, 192, 256, 384, 512, 768)");
+    }
     if (x_smem_bytes(g, s->xl.pairs, 3, true) > 65536) {
       delete s;
       return fail(KW_ERR_UNSUPPORTED, "x-row tile does not fit in LDS");
@@ -698,12 +740,28 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     const double dxs[3] = {d->dx, d->dy, g.dim == 3 ? d->dz : 1.0};
     const int nfull[3] = {g.Nx, g.Ny, g.Nz};
     const int nout[3] = {g.Nxh, g.Ny, g.Nz};
+    const bool want_ns = (d->record_flags & KW_REC_U_NON_STAGGERED) != 0;
     for (int a = 0; a < g.dim; ++a) {
       float2* dp = s->alloc<float2>(nout[a]);
       float2* dn = s->alloc<float2>(nout[a]);
-      launch_dvec(st0, dp, dn, nfull[a], nout[a], dxs[a]);
+      float2* sh = want_ns ? s->alloc<float2>(nout[a]) : nullptr;
+      launch_dvec(st0, dp, dn, sh, nfull[a], nout[a], dxs[a]);
       s->T.dpos[a] = dp;
       s->T.dneg[a] = dn;
+      s->T.shift[a] = sh;
+    }
+    if (want_ns) {
+      // x shift / Nx for the x C2R (the y and z shifts go through the spectral pass, 1/N folded in)
+      std::vector<float2> hx((size_t)g.Nxh);
+      HIPCHK(hipMemcpyAsync(hx.data(), s->T.shift[0], hx.size() * sizeof(float2), hipMemcpyDeviceToHost, st0));
+      HIPCHK(hipStreamSynchronize(st0));
+      for (auto& v : hx) {
+        v.x /= (float)g.Nx;
+        v.y /= (float)g.Nx;
+      }
+      s->ns_xmult = s->upload(hx.data(), hx.size());
+      s->ns_spec = s->alloc<float2>((size_t)s->nspec());
+      s->ns_field = s->alloc<float>((size_t)N);
     }
     OpParams op{};
     op.dx[0] = dxs[0];
@@ -817,6 +875,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
       if (rf & KW_REC_U_RMS) s->u_rms[a] = s->alloc<double>(2 * nsz);
       if (rf & KW_REC_U_MAX_ALL) s->u_max_all[a] = s->alloc<float>((size_t)N);
       if (rf & KW_REC_U_MIN_ALL) s->u_min_all[a] = s->alloc<float>((size_t)N);
+      if (rf & KW_REC_U_NON_STAGGERED) s->u_ns[a] = s->alloc<float>(nsz * (size_t)s->nrec_total);
     }
 
     // sources (point lists share the row CSR construction)
@@ -928,6 +987,7 @@ static const void* output_ptr(const kw_sim* s, int id, size_t* bytes, bool* is_r
       case KW_OUT_UX_MAX_ALL: *bytes = N * 4; return s->u_max_all[a];
       case KW_OUT_UX_MIN_ALL: *bytes = N * 4; return s->u_min_all[a];
       case KW_OUT_UX_FINAL: *bytes = N * 4; return (s->rec & KW_REC_U_FINAL) ? s->u[a] : nullptr;
+      case KW_OUT_UX_NON_STAGGERED: *bytes = ns * nrec * 4; return s->u_ns[a];
     }
     return nullptr;
   }

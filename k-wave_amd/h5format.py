@@ -163,7 +163,7 @@ def run_file(inp: str, out: str, record=("p",), record_start_index: int = 1, dev
     return o
 
 
-BINARY_FLAG = {"p": "--p_raw", "u": "--u_raw"}
+BINARY_FLAG = {"p": "--p_raw", "u": "--u_raw", "u_non_staggered": "--u_non_staggered_raw"}
 
 
 def binary_options(record=("p",), record_start_index=None, device=None):

@@ -23,11 +23,18 @@ RECORD_FLAGS = {
     "p": 1 << 0, "p_max": 1 << 1, "p_min": 1 << 2, "p_rms": 1 << 3, "p_max_all": 1 << 4,
     "p_min_all": 1 << 5, "p_final": 1 << 6, "u": 1 << 7, "u_max": 1 << 8, "u_min": 1 << 9,
     "u_rms": 1 << 10, "u_max_all": 1 << 11, "u_min_all": 1 << 12, "u_final": 1 << 13,
+    "u_non_staggered": 1 << 14,
 }
 
 # output ids (kw_output_id)
 _OUT_P = {"p": 0, "p_max": 1, "p_min": 2, "p_rms": 3, "p_max_all": 4, "p_min_all": 5, "p_final": 6}
-_OUT_U_BASE = {"": 7, "_max": 10, "_min": 13, "_rms": 16, "_max_all": 19, "_min_all": 22, "_final": 25}
+_OUT_U_BASE = {"": 7, "_max": 10, "_min": 13, "_rms": 16, "_max_all": 19, "_min_all": 22, "_final": 25,
+               "_non_staggered": 28}
+
+
+def _time_series(name: str) -> bool:
+    """Outputs shaped (Nt_rec, Ns)."""
+    return name in ("p", "ux", "uy", "uz") or name.endswith("_non_staggered")
 
 EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
@@ -309,7 +316,7 @@ class Simulation:
             raise KWaveError(1, f"output {name} was not recorded")
         out = np.empty(nbytes // 4, dtype=np.float32)
         _check(self.lib, self.lib.kw_sim_fetch(self.handle, oid, out.ctypes.data, nbytes))
-        if name in ("p",) or name in ("ux", "uy", "uz"):
+        if _time_series(name):
             return out.reshape(self.nrec, self.ns)
         if name.endswith("_all") or name.endswith("_final"):
             return out.reshape(self.Nz, self.Ny, self.Nx)
@@ -423,7 +430,7 @@ class Server:
             raise KWaveError(1, f"output {name} of request {rid} is not available ({self.state(rid)})")
         out = np.empty(nbytes // 4, dtype=np.float32)
         _check(self.lib, self.lib.kw_server_fetch(self.handle, rid, oid, out.ctypes.data, nbytes))
-        if name in ("p", "ux", "uy", "uz"):
+        if _time_series(name):
             return out.reshape(nrec, ns)
         if name.endswith("_all") or name.endswith("_final"):
             return out.reshape(shape)

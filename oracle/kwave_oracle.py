@@ -49,7 +49,7 @@ WORKERS = -1  # scipy.fft workers: all cores
 
 RECORD_NAMES = (
     "p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
-    "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
+    "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final", "u_non_staggered",
 )
 
 
@@ -374,6 +374,19 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
     if "u" in rec:
         for a, nm in enumerate("xyz"[:dim]):
             out["u" + nm] = np.zeros((max(nrec, 0), ns))
+    if "u_non_staggered" in rec:
+        # record.x_shift_neg = ifftshift(exp(-1i kx_vec dx / 2))
+        # (kspaceFirstOrder_createStorageVariables.m:87-99); array axis of MATLAB axis a is 2 - a
+        d = (pb.dx, pb.dy, pb.dz)
+        shift_neg = []
+        for a in range(dim):
+            n = shape[2 - a]
+            v = np.exp(-1j * k_vec_unshifted(n, d[a]) * d[a] / 2)
+            sh = [1, 1, 1]
+            sh[2 - a] = n
+            shift_neg.append(v.reshape(sh))
+        for a, nm in enumerate("xyz"[:dim]):
+            out["u" + nm + "_non_staggered"] = np.zeros((max(nrec, 0), ns))
     p_rms_acc = np.zeros(ns)
     u_rms_acc = [np.zeros(ns) for _ in range(dim)]
 
@@ -468,6 +481,11 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
                 us = u[a].reshape(-1)[sens]
                 if "u" in rec:
                     out["u" + nm][fi - 1] = us
+                if "u_non_staggered" in rec:
+                    # real(ifft(shift_neg .* fft(u, [], axis), [], axis)) (extractSensorData.m:39-55)
+                    ax = 2 - a
+                    ush = np.real(np.fft.ifft(shift_neg[a] * np.fft.fft(u[a], axis=ax), axis=ax))
+                    out["u" + nm + "_non_staggered"][fi - 1] = ush.reshape(-1)[sens]
                 if "u_max" in rec:
                     k = "u" + nm + "_max"
                     out[k] = us.copy() if fi == 1 else np.maximum(out[k], us)

@@ -18,7 +18,7 @@ def test_binary_matches_in_process_abi_and_oracle(kw, oracle, tmp_path):
     f = cases.case_3d(Nt=40)
     inp, out = str(tmp_path / "in.h5"), str(tmp_path / "out.h5")
     h5format.write_input(inp, f)
-    rec = ("p", "p_max", "p_rms", "u_final", "p_final")
+    rec = ("p", "p_max", "p_rms", "u_final", "p_final", "u_non_staggered")
     r = h5format.run_binary(inp, out, rec, record_start_index=5)
     assert r.returncode == 0, r.stderr
     o = h5format.read_file(out)
@@ -30,6 +30,7 @@ def test_binary_matches_in_process_abi_and_oracle(kw, oracle, tmp_path):
     assert o["p"].shape == (1, 40 - 5 + 1, ns)
     assert o["p_max"].shape == (1, 1, ns)
     assert o["p_final"].shape == (24, 28, 32)
+    assert o["uz_non_staggered"].shape == (1, 40 - 5 + 1, ns)   # /u?_non_staggered (3DC.m:706-711)
     direct = kw.run(f, record=rec, record_start_index=5)
     for k in direct:
         np.testing.assert_array_equal(o[k].reshape(direct[k].shape), direct[k])  # same kernels: bit-exact

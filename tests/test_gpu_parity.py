@@ -178,3 +178,21 @@ def test_c4_reduced(kw, oracle):
     """C4's branch (three-layer slab, y=1.1, p additive patch, receiver plane) on 96^3."""
     f = kw.configs.c4(Nt=60, N=96)
     compare(kw, oracle, f, ("p",), "c4_reduced")
+
+
+@pytest.mark.parametrize("case", ["2d", "3d_plane", "3d_pencil", "3d_p0_start"])
+def test_u_non_staggered(kw, oracle, case):
+    """--u_non_staggered_raw: u shifted to the unstaggered points, per axis."""
+    if case == "2d":
+        f = cases.case_2d_p0_lossless(N=64, Nt=40, het=True)
+        rec = ("u_non_staggered", "u")
+    elif case == "3d_plane":
+        f = cases.case_3d(N=(48, 32, 32), Nt=30, pml=4, source="uxyz")
+        rec = ("u_non_staggered", "p")
+    elif case == "3d_pencil":
+        f = cases.case_3d(N=(32, 48, 40), Nt=30, pml=4, source="uxyz")
+        rec = ("u_non_staggered",)
+    else:
+        f = cases.case_3d(N=(32, 32, 48), Nt=20, pml=4, p0=True, source=None)
+        rec = ("u_non_staggered", "u")
+    compare(kw, oracle, f, rec, f"u_non_staggered_{case}", start=3 if case == "2d" else 1)

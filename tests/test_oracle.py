@@ -385,3 +385,30 @@ def test_stokes_attenuation(kw, oracle):
     measured = np.log(P[band, 0] / P[band, 1]) / (200 * dx)
     expected = oracle.db2neper(a0, 2) * (2 * np.pi * freq[band]) ** 2
     np.testing.assert_allclose(measured, expected, rtol=0.05)
+
+
+def test_u_non_staggered_is_half_cell_shift(oracle):
+    """u_non_staggered = real(ifft(exp(-i k d/2) fft(u))) moves the staggered velocity
+    (at x + dx/2) back to the grid points: on a smooth field it matches the two-point
+    average (u(x - dx) + u(x)) / 2 to second order, and not the opposite-side average
+    (a sign mistake in the shift).  Also exact on the Nyquist-free closed form."""
+    import cases
+    f = cases.case_2d_p0_lossless(N=64, Nt=12, het=True)
+    n = 64
+    f["sensor_mask_index"] = np.arange(1, n * n + 1, dtype=np.uint64).reshape(1, 1, -1)
+    out = oracle.run(f, record=("u", "u_non_staggered"))
+    for a, nm in enumerate("xy"):
+        u = out["u" + nm][-1].reshape(n, n)            # (y, x) C order
+        ush = out["u" + nm + "_non_staggered"][-1].reshape(n, n)
+        ax = 1 - a
+        back = 0.5 * (np.roll(u, 1, axis=ax) + u)
+        fwd = 0.5 * (u + np.roll(u, -1, axis=ax))
+        e_back = np.linalg.norm(ush - back) / np.linalg.norm(ush)
+        e_fwd = np.linalg.norm(ush - fwd) / np.linalg.norm(ush)
+        assert e_back < 0.05 and e_fwd > 4 * e_back, (nm, e_back, e_fwd)
+    # closed form: a band-limited sinusoid shifted by -dx/2 exactly
+    x = np.arange(n)
+    u = np.cos(2 * np.pi * 5 * (x + 0.5) / n + 0.3)    # samples of cos at the staggered points
+    k = oracle.k_vec_unshifted(n, 1.0)
+    ush = np.real(np.fft.ifft(np.exp(-1j * k / 2) * np.fft.fft(u)))
+    assert np.allclose(ush, np.cos(2 * np.pi * 5 * x / n + 0.3), atol=1e-12)
