@@ -61,11 +61,15 @@ kw_status kwio_write_root_attributes(kwio_file* f, const char* file_type, const 
 
 /* Dataset shape (C order, padded to 3 dims with leading 1s) and type. */
 kw_status kwio_info(kwio_file* f, const char* name, int* dtype, uint64_t dims[3]);
+/* Rank (up to 8) and C-order dimensions of a dataset (cuboid-corner outputs
+ * are 4-D: (Nt, cz, cy, cx) == MATLAB [cx, cy, cz, Nt]). */
+kw_status kwio_info_nd(kwio_file* f, const char* name, int* dtype, int* rank, uint64_t dims[8]);
 kw_status kwio_read(kwio_file* f, const char* name, void* dst, size_t bytes);
 int kwio_exists(kwio_file* f, const char* name);
 
-/* Names of the root-level datasets, '\n'-separated; returns the byte count
- * needed (call with cap 0 to size the buffer). */
+/* Names of every dataset with its group path ("p", "p/1", ...),
+ * '\n'-separated; returns the byte count needed (call with cap 0 to size the
+ * buffer). */
 size_t kwio_list(kwio_file* f, char* buf, size_t cap);
 
 /* Thread-local message of the last kwio_* / kw_run_file failure. */

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <ctime>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -89,41 +90,55 @@ int kwio_exists(kwio_file* f, const char* name) {
   return H5Lexists(f->id, name, H5P_DEFAULT) > 0 ? 1 : 0;
 }
 
-kw_status kwio_write(kwio_file* f, const char* name, int dtype, const uint64_t dims_in[3], const void* data,
-                     int compression_level) {
+// Rank-3 datasets follow writeMatrix.m's chunking; rank 4 (per-cuboid time
+// series, (Nt, cz, cy, cx)) is chunked one time step per chunk.  Names with a
+// '/' create their groups ("p/1": dataset 1 in group p).
+static kw_status write_nd(kwio_file* f, const char* name, int dtype, int rank, const uint64_t* dims_in,
+                          const void* data, int compression_level) {
   if (!f || !name || !dims_in || !data) return iofail(KW_ERR_INVALID, "null argument");
   if (dtype != KWIO_FLOAT32 && dtype != KWIO_UINT64) return iofail(KW_ERR_INVALID, "dtype %d", dtype);
-  hsize_t dims[3] = {dims_in[0], dims_in[1], dims_in[2]};
-  for (int i = 0; i < 3; ++i)
+  if (rank != 3 && rank != 4) return iofail(KW_ERR_INVALID, "%s: rank %d", name, rank);
+  hsize_t dims[4] = {1, 1, 1, 1};
+  for (int i = 0; i < rank; ++i) dims[i] = dims_in[i];
+  for (int i = 0; i < rank; ++i)
     if (dims[i] == 0) return iofail(KW_ERR_INVALID, "%s: zero dimension", name);
   // chunking of writeMatrix.m:56-110, in C order (d0, d1, d2) == MATLAB [d2, d1, d0]
-  hsize_t chunk[3] = {1, 1, 1};
-  const hsize_t n = dims[0] * dims[1] * dims[2];
-  const hsize_t one_mb = 1024 * 1024 / 8;
+  hsize_t chunk[4] = {1, 1, 1, 1};
   int comp = compression_level;
-  if (dims[0] > 1) {  // 3-D
+  if (rank == 4) {
     chunk[1] = dims[1];
     chunk[2] = dims[2];
-  } else if (dims[1] > 1 && dims[2] > 1) {  // 2-D
-    chunk[2] = dims[2];
-  } else {  // vector or scalar
-    int axis = dims[2] > 1 ? 2 : (dims[1] > 1 ? 1 : 2);
-    if (n > one_mb) {
-      chunk[axis] = one_mb;
-    } else {
-      comp = 0;
-      chunk[axis] = dims[axis];
+    chunk[3] = dims[3];
+  } else {
+    const hsize_t n = dims[0] * dims[1] * dims[2];
+    const hsize_t one_mb = 1024 * 1024 / 8;
+    if (dims[0] > 1) {  // 3-D
+      chunk[1] = dims[1];
+      chunk[2] = dims[2];
+    } else if (dims[1] > 1 && dims[2] > 1) {  // 2-D
+      chunk[2] = dims[2];
+    } else {  // vector or scalar
+      int axis = dims[2] > 1 ? 2 : (dims[1] > 1 ? 1 : 2);
+      if (n > one_mb) {
+        chunk[axis] = one_mb;
+      } else {
+        comp = 0;
+        chunk[axis] = dims[axis];
+      }
     }
   }
   SilenceH5 quiet;
   if (H5Lexists(f->id, name, H5P_DEFAULT) > 0) H5Ldelete(f->id, name, H5P_DEFAULT);
-  hid_t space = H5Screate_simple(3, dims, nullptr);
+  hid_t space = H5Screate_simple(rank, dims, nullptr);
   hid_t dcpl = H5Pcreate(H5P_DATASET_CREATE);
-  H5Pset_chunk(dcpl, 3, chunk);
+  H5Pset_chunk(dcpl, rank, chunk);
   if (comp > 0) H5Pset_deflate(dcpl, (unsigned)comp);
+  hid_t lcpl = H5Pcreate(H5P_LINK_CREATE);
+  H5Pset_create_intermediate_group(lcpl, 1);
   hid_t ftype = dtype == KWIO_FLOAT32 ? H5T_IEEE_F32LE : H5T_STD_U64LE;
   hid_t mtype = dtype == KWIO_FLOAT32 ? H5T_NATIVE_FLOAT : H5T_NATIVE_UINT64;
-  hid_t ds = H5Dcreate2(f->id, name, ftype, space, H5P_DEFAULT, dcpl, H5P_DEFAULT);
+  hid_t ds = H5Dcreate2(f->id, name, ftype, space, lcpl, dcpl, H5P_DEFAULT);
+  H5Pclose(lcpl);
   kw_status st = KW_OK;
   if (ds < 0 || H5Dwrite(ds, mtype, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) < 0) {
     st = iofail(KW_ERR_IO, "cannot write dataset %s", name);
@@ -136,6 +151,11 @@ kw_status kwio_write(kwio_file* f, const char* name, int dtype, const uint64_t d
   H5Pclose(dcpl);
   H5Sclose(space);
   return st;
+}
+
+kw_status kwio_write(kwio_file* f, const char* name, int dtype, const uint64_t dims[3], const void* data,
+                     int compression_level) {
+  return write_nd(f, name, dtype, 3, dims, data, compression_level);
 }
 
 kw_status kwio_write_root_attributes(kwio_file* f, const char* file_type, const char* description) {
@@ -183,12 +203,40 @@ kw_status kwio_info(kwio_file* f, const char* name, int* dtype, uint64_t dims[3]
   return st;
 }
 
+kw_status kwio_info_nd(kwio_file* f, const char* name, int* dtype, int* rank, uint64_t dims[8]) {
+  if (!f || !name || !rank || !dims) return iofail(KW_ERR_INVALID, "null argument");
+  SilenceH5 quiet;
+  hid_t ds = H5Dopen2(f->id, name, H5P_DEFAULT);
+  if (ds < 0) return iofail(KW_ERR_IO, "dataset %s not found", name);
+  hid_t sp = H5Dget_space(ds);
+  const int nd = H5Sget_simple_extent_ndims(sp);
+  kw_status st = KW_OK;
+  if (nd < 0 || nd > 8) {
+    st = iofail(KW_ERR_IO, "dataset %s has %d dimensions", name, nd);
+  } else {
+    hsize_t d[8];
+    H5Sget_simple_extent_dims(sp, d, nullptr);
+    *rank = nd;
+    for (int i = 0; i < nd; ++i) dims[i] = d[i];
+    hid_t ty = H5Dget_type(ds);
+    H5T_class_t cls = H5Tget_class(ty);
+    if (dtype) *dtype = cls == H5T_FLOAT ? KWIO_FLOAT32 : KWIO_UINT64;
+    if (cls != H5T_FLOAT && cls != H5T_INTEGER) st = iofail(KW_ERR_IO, "dataset %s: unsupported type", name);
+    H5Tclose(ty);
+  }
+  H5Sclose(sp);
+  H5Dclose(ds);
+  return st;
+}
+
 kw_status kwio_read(kwio_file* f, const char* name, void* dst, size_t bytes) {
-  int dtype = 0;
-  uint64_t dims[3];
-  kw_status st = kwio_info(f, name, &dtype, dims);
+  int dtype = 0, rank = 0;
+  uint64_t dims[8];
+  kw_status st = kwio_info_nd(f, name, &dtype, &rank, dims);
   if (st) return st;
-  const size_t need = (size_t)(dims[0] * dims[1] * dims[2]) * (dtype == KWIO_FLOAT32 ? 4 : 8);
+  uint64_t count = 1;
+  for (int i = 0; i < rank; ++i) count *= dims[i];
+  const size_t need = (size_t)count * (dtype == KWIO_FLOAT32 ? 4 : 8);
   if (bytes != need) return iofail(KW_ERR_INVALID, "%s: %zu bytes requested, dataset holds %zu", name, bytes, need);
   SilenceH5 quiet;
   hid_t ds = H5Dopen2(f->id, name, H5P_DEFAULT);
@@ -198,15 +246,15 @@ kw_status kwio_read(kwio_file* f, const char* name, void* dst, size_t bytes) {
   return e < 0 ? iofail(KW_ERR_IO, "cannot read %s", name) : KW_OK;
 }
 
-static herr_t list_cb(hid_t, const char* name, const H5L_info_t*, void* op) {
-  static_cast<std::string*>(op)->append(name).append("\n");
+static herr_t list_cb(hid_t, const char* name, const H5O_info_t* info, void* op) {
+  if (info->type == H5O_TYPE_DATASET) static_cast<std::string*>(op)->append(name).append("\n");
   return 0;
 }
 
 size_t kwio_list(kwio_file* f, char* buf, size_t cap) {
   if (!f) return 0;
   std::string all;
-  H5Literate(f->id, H5_INDEX_NAME, H5_ITER_INC, nullptr, list_cb, &all);
+  H5Ovisit(f->id, H5_INDEX_NAME, H5_ITER_INC, list_cb, &all);
   if (buf && cap > 0) {
     size_t n = std::min(cap - 1, all.size());
     memcpy(buf, all.data(), n);
@@ -388,9 +436,10 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
       return iofail(KW_ERR_UNSUPPORTED, "input sets %s, which this solver does not run", n);
     }
   }
-  if (R.u64("sensor_mask_type")) {
+  const uint64_t mask_type = R.u64("sensor_mask_type");
+  if (mask_type > 1) {
     kwio_close(in);
-    return iofail(KW_ERR_UNSUPPORTED, "cuboid-corner sensor masks (sensor_mask_type 1) are not supported");
+    return iofail(KW_ERR_INVALID, "sensor_mask_type %llu", (unsigned long long)mask_type);
   }
   kw_sim_desc d;
   memset(&d, 0, sizeof d);
@@ -443,7 +492,49 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
     if (d.uy_source_flag) d.uy_source_input = R.fptr("uy_source_input");
     if (d.uz_source_flag) d.uz_source_input = R.fptr("uz_source_input");
   }
-  d.sensor_mask_index = R.uptr("sensor_mask_index", &d.sensor_npoints);
+  // Cuboid corners (sensor_mask_type 1, saveToDisk.m:357-378): columns
+  // [x1 y1 z1 x2 y2 z2] of 1-based inclusive corners.  The solver records the
+  // union of their points as a binary mask; the outputs are split per cuboid
+  // when written (kspaceFirstOrder3DC.m:714-767 reads /p/<k> ...).
+  struct Cuboid {
+    uint64_t lo[3], hi[3];
+    std::vector<size_t> pos;  // position of each point (x fastest) in the union
+  };
+  std::vector<Cuboid> cubs;
+  std::vector<uint64_t> cub_union;
+  if (mask_type == 1) {
+    uint64_t n6 = 0;
+    const uint64_t* c = R.uptr("sensor_mask_corners", &n6);
+    if (!R.st && (!c || n6 == 0 || n6 % 6)) R.st = iofail(KW_ERR_INVALID, "sensor_mask_corners must hold 6 x Ncuboid values");
+    const uint64_t Nd[3] = {d.Nx, d.Ny, d.Nz};
+    for (uint64_t k = 0; !R.st && k < n6 / 6; ++k) {
+      Cuboid q;
+      for (int a = 0; a < 3; ++a) {
+        q.lo[a] = c[6 * k + a];
+        q.hi[a] = c[6 * k + 3 + a];
+        if (q.lo[a] < 1 || q.lo[a] > q.hi[a] || q.hi[a] > Nd[a])
+          R.st = iofail(KW_ERR_INVALID, "cuboid %llu: corners outside the grid", (unsigned long long)(k + 1));
+      }
+      cubs.push_back(std::move(q));
+    }
+    for (const Cuboid& q : cubs)
+      for (uint64_t z = q.lo[2]; z <= q.hi[2]; ++z)
+        for (uint64_t y = q.lo[1]; y <= q.hi[1]; ++y)
+          for (uint64_t x = q.lo[0]; x <= q.hi[0]; ++x) cub_union.push_back(x + d.Nx * ((y - 1) + d.Ny * (z - 1)));
+    std::sort(cub_union.begin(), cub_union.end());
+    cub_union.erase(std::unique(cub_union.begin(), cub_union.end()), cub_union.end());
+    for (Cuboid& q : cubs)
+      for (uint64_t z = q.lo[2]; z <= q.hi[2]; ++z)
+        for (uint64_t y = q.lo[1]; y <= q.hi[1]; ++y)
+          for (uint64_t x = q.lo[0]; x <= q.hi[0]; ++x)
+            q.pos.push_back((size_t)(std::lower_bound(cub_union.begin(), cub_union.end(),
+                                                      x + d.Nx * ((y - 1) + d.Ny * (z - 1))) -
+                                     cub_union.begin()));
+    d.sensor_mask_index = cub_union.data();
+    d.sensor_npoints = cub_union.size();
+  } else {
+    d.sensor_mask_index = R.uptr("sensor_mask_index", &d.sensor_npoints);
+  }
   d.record_flags = o->record_flags ? o->record_flags : (uint64_t)KW_REC_P;
   d.record_start_index = o->record_start_index ? o->record_start_index : 1;
   kwio_close(in);
@@ -523,6 +614,27 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
     buf.resize(bytes / 4);
     if ((st = kw_sim_fetch(sim, id, buf.data(), bytes))) {
       iofail(st, "%s", kw_last_error());
+      return;
+    }
+    if (!cubs.empty() && kind != 2) {
+      // per cuboid: /name/<k> of (Nt_rec, cz, cy, cx) or (cz, cy, cx)
+      std::vector<float> part;
+      for (size_t k = 0; k < cubs.size() && !st; ++k) {
+        const Cuboid& q = cubs[k];
+        const uint64_t cd[3] = {q.hi[2] - q.lo[2] + 1, q.hi[1] - q.lo[1] + 1, q.hi[0] - q.lo[0] + 1};
+        const size_t np = q.pos.size();
+        const uint64_t steps = kind == 0 ? nrec : 1;
+        part.resize(np * steps);
+        for (uint64_t t = 0; t < steps; ++t)
+          for (size_t j = 0; j < np; ++j) part[t * np + j] = buf[t * ns + q.pos[j]];
+        const std::string dn = name + "/" + std::to_string(k + 1);
+        if (kind == 0) {
+          const uint64_t dims4[4] = {nrec, cd[0], cd[1], cd[2]};
+          if (nrec) st = write_nd(out, dn.c_str(), KWIO_FLOAT32, 4, dims4, part.data(), o->compression_level);
+        } else {
+          st = write_nd(out, dn.c_str(), KWIO_FLOAT32, 3, cd, part.data(), o->compression_level);
+        }
+      }
       return;
     }
     uint64_t dims[3];

@@ -241,7 +241,21 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
         f["u_source_many"] = _scalar(many, np.uint64)
         f["u_source_index"] = _col(idx)
 
-    # sensor (binary mask only: saveToDisk.m:35-37)
+    # sensor: binary mask, or cuboid corners (saveToDisk.m:35-37, 355-378)
+    if sensor.get("corners") is not None:
+        c = np.asarray(sensor["corners"], dtype=np.int64)
+        c = c.reshape(c.shape[0], -1)
+        if dim == 2:
+            if c.shape[0] != 4:
+                raise ValueError("2-D cuboid corners are [x1; y1; x2; y2] columns")
+            c6 = np.ones((6, c.shape[1]), dtype=np.int64)       # z corners 1 (saveToDisk.m:359-366)
+            c6[0], c6[1], c6[3], c6[4] = c[0], c[1], c[2], c[3]
+            c = c6
+        elif c.shape[0] != 6:
+            raise ValueError("3-D cuboid corners are [x1; y1; z1; x2; y2; z2] columns")
+        f["sensor_mask_type"] = _scalar(1, np.uint64)
+        f["sensor_mask_corners"] = np.ascontiguousarray(c.T[None].astype(np.uint64))   # [6, K] -> (1, K, 6)
+        return f
     mask = np.asarray(sensor["mask"])
     if mask.shape != shape:
         raise ValueError("sensor.mask must be a binary grid mask of the grid's shape")

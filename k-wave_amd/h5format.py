@@ -29,7 +29,7 @@ BINARY_PATH = os.path.join(PKG, "bin", "kspaceFirstOrder-HIP")
 KWIO_FLOAT32, KWIO_UINT64 = 0, 1
 
 IO_EXPORTED = ("kw_run_file", "kwio_open", "kwio_close", "kwio_write", "kwio_write_root_attributes",
-               "kwio_info", "kwio_read", "kwio_exists", "kwio_list", "kwio_read_string_attribute",
+               "kwio_info", "kwio_info_nd", "kwio_read", "kwio_exists", "kwio_list", "kwio_read_string_attribute",
                "kwio_last_error")
 
 
@@ -58,6 +58,8 @@ def load_io_library(path: str = IO_LIB_PATH):
     lib.kwio_info.argtypes = [vp, cp, C.POINTER(C.c_int), u64p]
     lib.kwio_read.argtypes = [vp, cp, vp, C.c_size_t]
     lib.kwio_exists.argtypes = [vp, cp]
+    lib.kwio_info_nd.argtypes = [vp, cp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
+    lib.kwio_info_nd.restype = C.c_int
     lib.kwio_list.argtypes = [vp, C.c_char_p, C.c_size_t]
     lib.kwio_list.restype = C.c_size_t
     lib.kwio_read_string_attribute.argtypes = [vp, cp, cp, C.c_char_p, C.c_size_t]
@@ -108,10 +110,12 @@ class H5File:
         return [s for s in buf.value.decode().split("\n") if s]
 
     def info(self, name: str):
-        dt = C.c_int()
-        dims = (C.c_uint64 * 3)()
-        _chk(self.lib, self.lib.kwio_info(self.h, name.encode(), C.byref(dt), dims))
-        return dt.value, tuple(int(d) for d in dims)
+        """(dtype, C-order dims): 3-D for the k-Wave datasets, 4-D for per-cuboid series."""
+        dt, rank = C.c_int(), C.c_int()
+        dims = (C.c_uint64 * 8)()
+        _chk(self.lib, self.lib.kwio_info_nd(self.h, name.encode(), C.byref(dt), C.byref(rank), dims))
+        d = tuple(int(dims[i]) for i in range(rank.value))
+        return dt.value, (1,) * (3 - len(d)) + d if len(d) < 3 else d
 
     def read(self, name: str) -> np.ndarray:
         dt, dims = self.info(name)

@@ -247,6 +247,9 @@ def build_desc(f: dict, record=("p",), record_start_index: int = 1, device: int 
         for k in ("ux", "uy", "uz"):
             if _int(f.get(k + "_source_flag")):
                 setattr(d, k + "_source_input", ptr_f(f[k + "_source_input"]))
+    if _int(f.get("sensor_mask_type")) == 1:
+        raise ValueError("cuboid-corner sensor masks are split per cuboid by the file path (kw_run_file / "
+                         "kspaceFirstOrder-HIP); in process, pass the cuboids' points as a binary mask")
     sm = f["sensor_mask_index"]
     d.sensor_npoints = np.asarray(sm).size
     d.sensor_mask_index = ptr_u(sm)

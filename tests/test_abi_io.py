@@ -213,7 +213,10 @@ def test_binary_missing_input_file(kw, tmp_path):
 
 @pytest.mark.parametrize("mutate,msg", [
     (lambda f: f.update(elastic_flag=np.array([[[1]]], dtype=np.uint64)), "elastic_flag"),
-    (lambda f: f.update(sensor_mask_type=np.array([[[1]]], dtype=np.uint64)), "cuboid"),
+    (lambda f: f.update(sensor_mask_type=np.array([[[1]]], dtype=np.uint64)), "sensor_mask_corners"),
+    (lambda f: f.update(sensor_mask_type=np.array([[[1]]], dtype=np.uint64),
+                        sensor_mask_corners=np.array([[[2, 2, 1, 70, 5, 1]]], dtype=np.uint64)), "outside the grid"),
+    (lambda f: f.update(sensor_mask_type=np.array([[[2]]], dtype=np.uint64)), "sensor_mask_type"),
     (lambda f: f.pop("c0"), "c0"),
     (lambda f: f.update(p_source_input=np.zeros((1, 3, 1), dtype=np.float32)), "p_source_input"),
 ])
@@ -238,3 +241,17 @@ def test_binary_rejects_output_file_type(kw, tmp_path):
         h.attributes("output")
     r = h5.run_binary(p, str(tmp_path / "x.h5"))
     assert r.returncode == 1 and "file_type" in r.stderr
+
+
+def test_cuboid_corners_front_end_layout(kw):
+    """saveToDisk.m:355-378: corners [6, K] -> (1, K, 6) uint64; 2-D [4, K] gets z = 1."""
+    fe = kw.frontend
+    kg = fe.KWaveGrid(32, 1e-4, 24, 1e-4)
+    kg.set_time(5, 2e-8)
+    f = fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": np.ones((32, 24))},
+                             {"corners": np.array([[3, 10], [4, 2], [8, 20], [9, 7]])}, pml_size=4)
+    assert int(f["sensor_mask_type"].ravel()[0]) == 1 and "sensor_mask_index" not in f
+    assert f["sensor_mask_corners"].dtype == np.uint64 and f["sensor_mask_corners"].shape == (1, 2, 6)
+    assert f["sensor_mask_corners"][0].tolist() == [[3, 4, 1, 8, 9, 1], [10, 2, 1, 20, 7, 1]]
+    with pytest.raises(ValueError):
+        kw.native.build_desc(f)

@@ -74,3 +74,103 @@ def test_matlab_style_front_end(kw, oracle):
     ref = oracle.run(f, record=("p", "p_final"))
     assert cases.rel_l2(sd["p"], ref["p"].T) < 1e-4
     assert cases.rel_l2(sd["p_final"], ref["p_final"][0].T) < 1e-4
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_cuboid_corners_equal_binary_mask(kw, tmp_path, dim):
+    """Cuboid-corner sensors through the file path: /p/<k> etc. hold, per cuboid and in
+    [cx, cy, cz, Nt] order, exactly the binary-mask run's values at those points
+    (the reference's binary == cuboid invariant)."""
+    from kwave_amd import h5format
+
+    print(f"[gpu binary] cuboid corners vs binary mask, {dim}-D", flush=True)
+    if dim == 3:
+        f = cases.case_3d(N=(32, 28, 24), Nt=30)
+        shape = (32, 28, 24)
+        corners = np.array([[5, 4, 3, 12, 9, 7], [10, 8, 6, 20, 14, 10]]).T      # overlapping
+    else:
+        f = cases.case_2d_absorbing_psource(N=(60, 48), Nt=40)
+        shape = (60, 48)
+        corners = np.array([[10, 12, 30, 20], [25, 5, 40, 15]]).T
+    rec = ("p", "p_max", "p_rms", "u", "p_final")
+    fc = dict(f)
+    fc.pop("sensor_mask_index")
+    # the front end's corner layout (saveToDisk.m:355-378)
+    c6 = corners if dim == 3 else np.vstack([corners[0], corners[1], np.ones(2, int),
+                                             corners[2], corners[3], np.ones(2, int)])
+    fc["sensor_mask_type"] = np.array([[[1]]], dtype=np.uint64)
+    fc["sensor_mask_corners"] = np.ascontiguousarray(c6.T[None].astype(np.uint64))
+    mask = np.zeros(shape)
+    for k in range(corners.shape[1]):
+        c = c6[:, k] - 1
+        if dim == 3:
+            mask[c[0]:c[3] + 1, c[1]:c[4] + 1, c[2]:c[5] + 1] = 1
+        else:
+            mask[c[0]:c[3] + 1, c[1]:c[4] + 1] = 1
+    fb = dict(f)
+    idx = np.flatnonzero(mask.reshape(-1, order="F")) + 1
+    fb["sensor_mask_index"] = idx.astype(np.uint64).reshape(1, 1, -1)
+    outs = {}
+    for tag, ff in (("cub", fc), ("bin", fb)):
+        inp, out = str(tmp_path / f"{tag}.h5"), str(tmp_path / f"{tag}_out.h5")
+        h5format.write_input(inp, ff)
+        r = h5format.run_binary(inp, out, rec)
+        assert r.returncode == 0, r.stderr
+        outs[tag] = h5format.read_file(out)
+    ob, oc = outs["bin"], outs["cub"]
+    nt = int(np.ravel(f["Nt"])[0])
+    Nx, Ny = shape[0], shape[1]
+    for k in range(corners.shape[1]):
+        lo = c6[:3, k]
+        hi = c6[3:, k]
+        xs, ys, zs = (np.arange(lo[a], hi[a] + 1) for a in range(3))
+        Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")               # (cz, cy, cx), x fastest
+        lin = X + Nx * ((Y - 1) + Ny * (Z - 1))
+        col = np.searchsorted(idx, lin.reshape(-1))
+        cshape = (len(zs), len(ys), len(xs))
+        for name in ("p", "ux", "uy") + (("uz",) if dim == 3 else ()):
+            got = oc[f"{name}/{k + 1}"]
+            assert got.shape == (nt,) + cshape, (name, got.shape)
+            want = ob[name].reshape(nt, -1)[:, col].reshape((nt,) + cshape)
+            np.testing.assert_array_equal(got, want)
+        for name in ("p_max", "p_rms"):
+            got = oc[f"{name}/{k + 1}"]
+            assert got.shape == cshape
+            np.testing.assert_array_equal(got, ob[name].reshape(-1)[col].reshape(cshape))
+    np.testing.assert_array_equal(oc["p_final"], ob["p_final"])
+
+
+def test_time_reversal_through_the_contract(kw, oracle, tmp_path):
+    """Time reversal as kspaceFirstOrder3DC.m:437-460 rewrites it: the recorded sensor
+    data, flipped in time, drives a dirichlet p source on the sensor mask, and the
+    binary records p_final over the whole grid (the reconstruction)."""
+    from kwave_amd import h5format
+
+    print("[gpu binary] time reversal via dirichlet source + p_final", flush=True)
+    fe, geo = kw.frontend, kw.geometry
+    N, d = 96, 1e-4
+    kg = fe.KWaveGrid(N, d, N, d)
+    kg.set_time(260, 0.3 * d / 1500.0)
+    p0 = 2.0 * geo.make_disc(N, N, 40, 52, 6)
+    X, Y = np.meshgrid(np.arange(N) - N // 2, np.arange(N) - N // 2, indexing="ij")
+    ring = (np.abs(np.sqrt(X ** 2 + Y ** 2) - 34) < 0.5).astype(float)
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    fwd = fe.save_to_disk_dict(kg, med, {"p0": p0}, {"mask": ring}, pml_size=10)
+    rec = kw.run(fwd, record=("p",))["p"]                     # (Nt, Ns)
+    tr = fe.save_to_disk_dict(kg, med, {"p": rec.T[:, ::-1], "p_mask": ring, "p_mode": "dirichlet"},
+                              {"mask": np.ones((N, N))}, pml_size=10)
+    inp, out = str(tmp_path / "tr.h5"), str(tmp_path / "tr_out.h5")
+    h5format.write_input(inp, tr)
+    r = h5format.run_binary(inp, out, ("p_final",))
+    assert r.returncode == 0, r.stderr
+    got = h5format.read_file(out)["p_final"].reshape(N, N)      # (y, x)
+    ref = oracle.run(tr, record=("p_final",))["p_final"].reshape(N, N)
+    e = cases.rel_l2(got, ref)
+    print(f"    p_final rel L2 vs oracle {e:.2e}", flush=True)
+    assert e < 1e-4
+    # the reconstruction resembles the source distribution
+    p0s = np.asarray(fwd["p0_source_input"], dtype=np.float64).reshape(N, N)
+    inside = (np.sqrt(X ** 2 + Y ** 2) < 30).T
+    cc = np.corrcoef(got[inside], p0s[inside])[0, 1]
+    print(f"    correlation with p0 inside the ring {cc:.2f}", flush=True)
+    assert cc > 0.5
