@@ -159,6 +159,16 @@ kw_status kw_sim_fetch(kw_sim* sim, int output_id, void* host_dst, size_t bytes)
 /* Frees device memory and the handle. */
 kw_status kw_sim_destroy(kw_sim* sim);
 
+/* Checkpoint-restart (the binaries' --checkpoint_file, file_type "checkpoint",
+ * private/getH5Literals.m:48; release notes: "command line inputs for
+ * checkpoint-restart").  The state is the time index, the fields carried
+ * between steps and every running output; a handle created from the same
+ * description restores it with kw_sim_checkpoint_load and continues exactly as
+ * the uninterrupted run would. */
+size_t kw_sim_checkpoint_bytes(const kw_sim* sim);
+kw_status kw_sim_checkpoint_save(kw_sim* sim, void* host_dst, size_t bytes);
+kw_status kw_sim_checkpoint_load(kw_sim* sim, const void* host_src, size_t bytes);
+
 /* Thread-local description of the last failure on this thread. */
 const char* kw_last_error(void);
 

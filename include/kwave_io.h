@@ -33,6 +33,19 @@ typedef struct kw_run_options {
   int32_t reserved;
   /* filled in by kw_run_file: wall-clock seconds of each phase */
   double t_load, t_precompute, t_loop, t_save;
+  /* checkpoint-restart (--checkpoint_file, --checkpoint_interval,
+   * --checkpoint_timesteps; file_type "checkpoint", getH5Literals.m:48).
+   * If checkpoint_file names an existing checkpoint of this input, the run
+   * resumes from it.  Once checkpoint_interval seconds (> 0) of time loop or
+   * checkpoint_timesteps steps (> 0) have passed in this call, the state is
+   * saved there, checkpointed is set to 1 and no output file is written; the
+   * same call again continues.  A run that finishes writes the output file
+   * and deletes the checkpoint. */
+  const char* checkpoint_file;
+  double checkpoint_interval;
+  uint64_t checkpoint_timesteps;
+  int32_t checkpointed; /* out */
+  int32_t reserved2;
 } kw_run_options;
 
 /* Reads the input file, runs the simulation, writes the output file.

@@ -65,6 +65,59 @@ struct kwio_file {
   bool writable = false;
 };
 
+static bool file_exists(const char* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (f) std::fclose(f);
+  return f != nullptr;
+}
+
+// checkpoint file: root attributes with file_type "checkpoint" and the solver
+// state as one uint64 dataset
+static kw_status save_checkpoint(const char* path, kw_sim* sim) {
+  const size_t bytes = kw_sim_checkpoint_bytes(sim);
+  std::vector<uint64_t> words((bytes + 7) / 8);
+  kw_status st = kw_sim_checkpoint_save(sim, words.data(), bytes);
+  if (st) return iofail(st, "%s", kw_last_error());
+  const std::string tmp = std::string(path) + ".part";
+  kwio_file* f = nullptr;
+  if ((st = kwio_open(tmp.c_str(), 1, &f))) return st;
+  const uint64_t dims[3] = {1, 1, (uint64_t)words.size()};
+  const uint64_t nb[3] = {1, 1, 1};
+  const uint64_t b64 = bytes;
+  st = kwio_write(f, "kw_state", KWIO_UINT64, dims, words.data(), 0);
+  if (!st) st = kwio_write(f, "kw_state_bytes", KWIO_UINT64, nb, &b64, 0);
+  if (!st) st = kwio_write_root_attributes(f, "checkpoint", "Checkpoint of kspaceFirstOrder-HIP");
+  kwio_close(f);
+  if (st) return st;
+  if (std::rename(tmp.c_str(), path) != 0) return iofail(KW_ERR_IO, "cannot write checkpoint %s", path);
+  return KW_OK;
+}
+
+static kw_status load_checkpoint(const char* path, kw_sim* sim) {
+  kwio_file* f = nullptr;
+  kw_status st = kwio_open(path, 0, &f);
+  if (st) return st;
+  char ft[64] = {0};
+  if (kwio_read_string_attribute(f, "/", "file_type", ft, sizeof ft) || std::string(ft) != "checkpoint") {
+    kwio_close(f);
+    return iofail(KW_ERR_INVALID, "%s is not a checkpoint file (file_type '%s')", path, ft);
+  }
+  int dt = 0;
+  uint64_t dims[3];
+  uint64_t bytes = 0;
+  if ((st = kwio_read(f, "kw_state_bytes", &bytes, 8)) || (st = kwio_info(f, "kw_state", &dt, dims))) {
+    kwio_close(f);
+    return st;
+  }
+  std::vector<uint64_t> words(dims[0] * dims[1] * dims[2]);
+  st = kwio_read(f, "kw_state", words.data(), words.size() * 8);
+  kwio_close(f);
+  if (st) return st;
+  if (bytes > words.size() * 8) return iofail(KW_ERR_INVALID, "%s: truncated checkpoint", path);
+  if ((st = kw_sim_checkpoint_load(sim, words.data(), (size_t)bytes))) return iofail(st, "%s", kw_last_error());
+  return KW_OK;
+}
+
 extern "C" {
 
 kw_status kwio_open(const char* path, int mode, kwio_file** out) {
@@ -575,12 +628,45 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
            (unsigned long long)d.Ny, (unsigned long long)d.Nz, (unsigned long long)d.Nt);
     fflush(stdout);
   }
+  // resume from a checkpoint of this input
+  o->checkpointed = 0;
+  const bool ckpt = o->checkpoint_file && o->checkpoint_file[0];
+  if (ckpt && file_exists(o->checkpoint_file)) {
+    if ((st = load_checkpoint(o->checkpoint_file, sim))) {
+      kw_sim_destroy(sim);
+      return st;
+    }
+    if (o->verbose)
+      printf("  resumed from checkpoint %s at step %llu\n", o->checkpoint_file,
+             (unsigned long long)kw_sim_steps_done(sim));
+  }
+  const uint64_t first = kw_sim_steps_done(sim);
   const uint64_t chunk = d.Nt >= 10 ? d.Nt / 10 : d.Nt;
   while (kw_sim_steps_done(sim) < d.Nt) {
-    if ((st = kw_sim_run(sim, chunk, nullptr)) || (st = kw_sim_sync(sim))) {
+    uint64_t n = chunk;
+    if (ckpt && o->checkpoint_timesteps)
+      n = std::min<uint64_t>(n, first + o->checkpoint_timesteps - kw_sim_steps_done(sim));
+    if ((st = kw_sim_run(sim, n, nullptr)) || (st = kw_sim_sync(sim))) {
       iofail(st, "%s", kw_last_error());
       kw_sim_destroy(sim);
       return st;
+    }
+    const uint64_t done = kw_sim_steps_done(sim);
+    if (ckpt && done < d.Nt &&
+        ((o->checkpoint_timesteps && done - first >= o->checkpoint_timesteps) ||
+         (o->checkpoint_interval > 0 && now_s() - t2 >= o->checkpoint_interval))) {
+      st = save_checkpoint(o->checkpoint_file, sim);
+      kw_sim_destroy(sim);
+      if (st) return st;
+      o->checkpointed = 1;
+      o->t_load = t1 - t0;
+      o->t_precompute = t2 - t1;
+      o->t_loop = now_s() - t2;
+      o->t_save = 0;
+      if (o->verbose)
+        printf("  checkpoint saved to %s at step %llu of %llu\n", o->checkpoint_file, (unsigned long long)done,
+               (unsigned long long)d.Nt);
+      return KW_OK;
     }
     if (o->verbose >= 2) {
       printf("  %5.1f%% (%llu of %llu steps)\n", 100.0 * kw_sim_steps_done(sim) / d.Nt,
@@ -663,6 +749,7 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
   if (!st) st = kwio_write_root_attributes(out, "output", "Output data created by kspaceFirstOrder-HIP");
   kwio_close(out);
   kw_sim_destroy(sim);
+  if (!st && ckpt && file_exists(o->checkpoint_file)) std::remove(o->checkpoint_file);
   const double t4 = now_s();
   o->t_load = t1 - t0;
   o->t_precompute = t2 - t1;

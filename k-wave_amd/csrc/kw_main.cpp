@@ -21,11 +21,13 @@ static void usage(FILE* out) {
           "  --verbose <0..2>\n"
           "  --p_raw --p_max --p_min --p_rms --p_max_all --p_min_all --p_final\n"
           "  --u_raw --u_max --u_min --u_rms --u_max_all --u_min_all --u_final --u_non_staggered_raw\n"
-          "  (no output flag: --p_raw)\n");
+          "  (no output flag: --p_raw)\n"
+          "  --checkpoint_file <f> [--checkpoint_interval <s>] [--checkpoint_timesteps <n>]\n"
+          "                      stop and save after <s> seconds / <n> steps; run again to continue\n");
 }
 
 int main(int argc, char** argv) {
-  std::string in, out;
+  std::string in, out, ckpt_file;
   kw_run_options o;
   memset(&o, 0, sizeof o);
   o.record_start_index = 1;
@@ -74,8 +76,13 @@ int main(int argc, char** argv) {
     } else if (a == "--version") {
       printf("kspaceFirstOrder-HIP (gfx950), HDF5 file format 1.2\n");
       return 0;
-    } else if (a == "--copy_sensor_mask" || a == "--checkpoint_file" ||
-               a == "--checkpoint_interval" || a == "--benchmark") {
+    } else if (a == "--checkpoint_file") {
+      ckpt_file = val("--checkpoint_file");
+    } else if (a == "--checkpoint_interval") {
+      o.checkpoint_interval = atof(val("--checkpoint_interval"));
+    } else if (a == "--checkpoint_timesteps") {
+      o.checkpoint_timesteps = strtoull(val("--checkpoint_timesteps"), nullptr, 10);
+    } else if (a == "--copy_sensor_mask" || a == "--benchmark") {
       fprintf(stderr, "kspaceFirstOrder-HIP: option %s is not supported by this build\n", a.c_str());
       return 3;
     } else {
@@ -94,11 +101,13 @@ int main(int argc, char** argv) {
   printf("└───────────────────────────────────────────────────────────────┘\n");
   printf("  input  : %s\n  output : %s\n", in.c_str(), out.c_str());
   fflush(stdout);
+  if (!ckpt_file.empty()) o.checkpoint_file = ckpt_file.c_str();
   kw_status st = kw_run_file(in.c_str(), out.c_str(), &o);
   if (st != KW_OK) {
     fprintf(stderr, "kspaceFirstOrder-HIP: error %d: %s\n", (int)st, kwio_last_error());
     return 1;
   }
+  if (o.checkpointed) printf("  checkpoint saved to %s; run the same command again to continue\n", ckpt_file.c_str());
   printf("  data loading    : %8.3f s\n", o.t_load);
   printf("  pre-processing  : %8.3f s\n", o.t_precompute);
   printf("  time loop       : %8.3f s\n", o.t_loop);

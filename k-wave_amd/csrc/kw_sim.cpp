@@ -568,6 +568,62 @@ kw_status setup_source(kw_sim* s, SrcData& d, uint64_t flag, uint64_t mode, uint
 // =============================================================================
 // C ABI
 // =============================================================================
+// ---- checkpoint state: the time index, the fields carried between steps
+// (u, rho, the spectrum of p) and every running output.  Medium, operators
+// and sources are rebuilt from the input at restart.
+namespace {
+struct StateBuf {
+  void* p;
+  size_t bytes;
+};
+std::vector<StateBuf> state_buffers(const kw_sim* s) {
+  std::vector<StateBuf> v;
+  const size_t N = (size_t)s->g.N, ns = (size_t)std::max(s->ns, 1), nrec = (size_t)s->nrec_total;
+  auto add = [&](const void* p, size_t b) {
+    if (p) v.push_back({const_cast<void*>(p), b});
+  };
+  for (int a = 0; a < s->g.dim; ++a) {
+    add(s->u[a], N * 4);
+    add(s->rho[a], N * 4);
+  }
+  add(s->Pk, (size_t)s->nspec() * 8);
+  add(s->p_raw, ns * nrec * 4);
+  add(s->p_max, ns * 4);
+  add(s->p_min, ns * 4);
+  add(s->p_rms, ns * 16);
+  add(s->p_max_all, N * 4);
+  add(s->p_min_all, N * 4);
+  add(s->p_final, N * 4);
+  for (int a = 0; a < s->g.dim; ++a) {
+    add(s->u_raw[a], ns * nrec * 4);
+    add(s->u_max[a], ns * 4);
+    add(s->u_min[a], ns * 4);
+    add(s->u_rms[a], ns * 16);
+    add(s->u_max_all[a], N * 4);
+    add(s->u_min_all[a], N * 4);
+    add(s->u_ns[a], ns * nrec * 4);
+  }
+  return v;
+}
+constexpr uint64_t kCkptMagic = 0x31305450434b574bULL;  // "KWCKPT01"
+constexpr size_t kCkptHeader = 12;                      // uint64 words
+void ckpt_header(const kw_sim* s, uint64_t* h) {
+  h[0] = kCkptMagic;
+  h[1] = (uint64_t)s->g.Nx;
+  h[2] = (uint64_t)s->g.Ny;
+  h[3] = (uint64_t)s->g.Nz;
+  h[4] = (uint64_t)s->Nt;
+  h[5] = (uint64_t)s->t;
+  h[6] = (uint64_t)s->ns;
+  h[7] = s->rec;
+  h[8] = (uint64_t)s->start;
+  h[9] = (uint64_t)s->g.kxm;
+  h[10] = (uint64_t)state_buffers(s).size();
+  h[11] = 0;
+}
+}  // namespace
+
+
 extern "C" {
 
 const char* kw_last_error(void) { return g_err.c_str(); }
@@ -1041,6 +1097,58 @@ kw_status kw_sim_destroy(kw_sim* s) {
   (void)hipSetDevice(s->device);
   free_all(s);
   delete s;
+  return KW_OK;
+}
+
+size_t kw_sim_checkpoint_bytes(const kw_sim* s) {
+  if (!s) return 0;
+  size_t b = kCkptHeader * 8;
+  for (const StateBuf& x : state_buffers(s)) b += x.bytes;
+  return b;
+}
+
+kw_status kw_sim_checkpoint_save(kw_sim* s, void* host, size_t bytes) {
+  if (!s || !host) return fail(KW_ERR_INVALID, "null argument");
+  if (bytes != kw_sim_checkpoint_bytes(s)) return fail(KW_ERR_INVALID, "checkpoint needs %zu bytes", kw_sim_checkpoint_bytes(s));
+  try {
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->last ? s->last : s->stream));
+    uint64_t* h = static_cast<uint64_t*>(host);
+    ckpt_header(s, h);
+    char* p = static_cast<char*>(host) + kCkptHeader * 8;
+    for (const StateBuf& x : state_buffers(s)) {
+      HIPCHK(hipMemcpy(p, x.p, x.bytes, hipMemcpyDeviceToHost));
+      p += x.bytes;
+    }
+  } catch (const HipError& e) {
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  }
+  return KW_OK;
+}
+
+kw_status kw_sim_checkpoint_load(kw_sim* s, const void* host, size_t bytes) {
+  if (!s || !host) return fail(KW_ERR_INVALID, "null argument");
+  if (bytes != kw_sim_checkpoint_bytes(s)) return fail(KW_ERR_INVALID, "checkpoint size does not match this simulation");
+  const uint64_t* h = static_cast<const uint64_t*>(host);
+  uint64_t want[kCkptHeader];
+  ckpt_header(s, want);
+  if (h[0] != kCkptMagic) return fail(KW_ERR_INVALID, "not a checkpoint of this solver");
+  for (size_t i = 1; i < kCkptHeader; ++i)
+    if (i != 5 && h[i] != want[i])
+      return fail(KW_ERR_INVALID, "checkpoint was written for a different simulation (header word %zu)", i);
+  if (h[5] > (uint64_t)s->Nt) return fail(KW_ERR_INVALID, "checkpoint time index beyond Nt");
+  try {
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->last ? s->last : s->stream));
+    const char* p = static_cast<const char*>(host) + kCkptHeader * 8;
+    for (const StateBuf& x : state_buffers(s)) {
+      HIPCHK(hipMemcpy(x.p, p, x.bytes, hipMemcpyHostToDevice));
+      p += x.bytes;
+    }
+    s->t = (long long)h[5];
+  } catch (const HipError& e) {
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  }
   return KW_OK;
 }
 

@@ -37,7 +37,8 @@ class KwRunOptions(C.Structure):
     _fields_ = [("record_flags", C.c_uint64), ("record_start_index", C.c_uint64), ("device", C.c_int32),
                 ("verbose", C.c_int32), ("compression_level", C.c_int32), ("reserved", C.c_int32),
                 ("t_load", C.c_double), ("t_precompute", C.c_double), ("t_loop", C.c_double),
-                ("t_save", C.c_double)]
+                ("t_save", C.c_double), ("checkpoint_file", C.c_char_p), ("checkpoint_interval", C.c_double),
+                ("checkpoint_timesteps", C.c_uint64), ("checkpointed", C.c_int32), ("reserved2", C.c_int32)]
 
 
 _io = None
@@ -155,7 +156,9 @@ def read_file(path: str) -> dict:
 
 
 def run_file(inp: str, out: str, record=("p",), record_start_index: int = 1, device: int = 0,
-             verbose: int = 0) -> KwRunOptions:
+             verbose: int = 0, checkpoint_file: str | None = None, checkpoint_interval: float = 0.0,
+             checkpoint_timesteps: int = 0) -> KwRunOptions:
+    """kw_run_file; with a checkpoint file the call may stop early (o.checkpointed == 1)."""
     lib = load_io_library()
     o = KwRunOptions()
     for r in record:
@@ -163,6 +166,10 @@ def run_file(inp: str, out: str, record=("p",), record_start_index: int = 1, dev
     o.record_start_index = record_start_index
     o.device = device
     o.verbose = verbose
+    if checkpoint_file:
+        o.checkpoint_file = checkpoint_file.encode()
+        o.checkpoint_interval = float(checkpoint_interval)
+        o.checkpoint_timesteps = int(checkpoint_timesteps)
     _chk(lib, lib.kw_run_file(inp.encode(), out.encode(), C.byref(o)))
     return o
 
@@ -183,12 +190,12 @@ def binary_options(record=("p",), record_start_index=None, device=None):
 
 
 def run_binary(inp: str, out: str, record=("p",), record_start_index=None, device=None, binary=BINARY_PATH,
-               timeout=None) -> subprocess.CompletedProcess:
+               timeout=None, extra=()) -> subprocess.CompletedProcess:
     """Run the executable with LD_LIBRARY_PATH cleared, from its own directory
-    (kspaceFirstOrder3DC.m:544)."""
+    (kspaceFirstOrder3DC.m:544); `extra` appends options (e.g. checkpointing)."""
     env = dict(os.environ)
     env["LD_LIBRARY_PATH"] = ""
-    cmd = [binary, "-i", inp, "-o", out] + binary_options(record, record_start_index, device)
+    cmd = [binary, "-i", inp, "-o", out] + binary_options(record, record_start_index, device) + list(extra)
     return subprocess.run(cmd, cwd=os.path.dirname(binary), env=env, capture_output=True, text=True,
                           timeout=timeout)
 

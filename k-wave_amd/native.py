@@ -40,6 +40,7 @@ EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
     "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
     "kw_sim_footprint_bytes", "kw_diag_copy", "kw_sim_stream",
+    "kw_sim_checkpoint_bytes", "kw_sim_checkpoint_save", "kw_sim_checkpoint_load",
     "kw_server_create", "kw_server_submit", "kw_server_poll", "kw_server_wait", "kw_server_request_info",
     "kw_server_request_error", "kw_server_output_bytes", "kw_server_fetch", "kw_server_release",
     "kw_server_stats_get", "kw_server_destroy",

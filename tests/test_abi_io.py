@@ -199,8 +199,7 @@ def test_binary_usage_errors(kw, tmp_path):
     assert r.returncode == 2 and "usage" in r.stderr
     r = subprocess.run([h5.BINARY_PATH, "-i", "a", "-o", "b", "--bogus"], capture_output=True, text=True)
     assert r.returncode == 2 and "unknown option" in r.stderr
-    r = subprocess.run([h5.BINARY_PATH, "-i", "a", "-o", "b", "--checkpoint_file", "c.h5"], capture_output=True,
-                       text=True)
+    r = subprocess.run([h5.BINARY_PATH, "-i", "a", "-o", "b", "--benchmark"], capture_output=True, text=True)
     assert r.returncode == 3 and "not supported" in r.stderr
 
 

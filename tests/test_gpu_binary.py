@@ -174,3 +174,39 @@ def test_time_reversal_through_the_contract(kw, oracle, tmp_path):
     cc = np.corrcoef(got[inside], p0s[inside])[0, 1]
     print(f"    correlation with p0 inside the ring {cc:.2f}", flush=True)
     assert cc > 0.5
+
+
+@pytest.mark.parametrize("how", ["abi", "binary"])
+def test_checkpoint_restart_is_exact(kw, tmp_path, how):
+    """A run stopped at checkpoints and resumed (twice) gives bit-identical outputs
+    to the uninterrupted run; the finished run removes the checkpoint."""
+    from kwave_amd import h5format
+
+    print(f"[gpu binary] checkpoint-restart via {how}", flush=True)
+    f = cases.case_3d(N=(32, 28, 24), Nt=50, nonlinear=True, source="p", many=True)
+    rec = ("p", "p_max", "p_rms", "u_max", "p_final", "u_non_staggered")
+    inp = str(tmp_path / "in.h5")
+    h5format.write_input(inp, f)
+    ref_out, out, ck = str(tmp_path / "ref.h5"), str(tmp_path / "out.h5"), str(tmp_path / "ck.h5")
+    h5format.run_file(inp, ref_out, rec)
+    stops = 0
+    for _ in range(4):
+        if how == "abi":
+            o = h5format.run_file(inp, out, rec, checkpoint_file=ck, checkpoint_timesteps=20)
+            stopped = o.checkpointed == 1
+        else:
+            r = h5format.run_binary(inp, out, rec, extra=["--checkpoint_file", ck, "--checkpoint_timesteps", "20"])
+            assert r.returncode == 0, r.stderr
+            stopped = "checkpoint saved" in r.stdout
+        if not stopped:
+            break
+        stops += 1
+        assert os.path.exists(ck) and not os.path.exists(out)
+        with h5format.H5File(ck) as h:
+            assert h.attribute("file_type") == "checkpoint"
+    assert stops == 2                       # 50 steps in calls of 20: stop at 20, 40, finish
+    assert not os.path.exists(ck)
+    a, b = h5format.read_file(ref_out), h5format.read_file(out)
+    assert sorted(a) == sorted(b)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
