@@ -91,9 +91,9 @@ def _updates(f: dict) -> int:
     return int(np.prod([int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz", "Nt")]))
 
 
-def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int):
+def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int, autotune: bool = False):
     """Submit reqs at the given arrival offsets (s) to a fresh server; wait for all."""
-    with kw.Server(max_active=max_active, steps_per_quantum=quantum) as srv:
+    with kw.Server(max_active=max_active, steps_per_quantum=quantum, autotune=autotune) as srv:
         t0 = time.perf_counter()
         ids = []
         for f, ta in zip(reqs, arrivals):
@@ -112,7 +112,8 @@ def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int):
         peak = srv.stats()["hbm_peak_reserved_bytes"]
     lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
     upd = sum(_updates(f) for f in reqs)
-    res = {"max_active": max_active, "makespan_s": round(makespan, 3), "updates_per_s": upd / makespan,
+    res = {"max_active": max_active, "autotune": autotune, "makespan_s": round(makespan, 3),
+           "updates_per_s": upd / makespan,
            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
            "hbm_peak_reserved_GiB": round(peak / 2 ** 30, 2)}
@@ -134,15 +135,17 @@ def serving_leg(kw, M: int, rates, max_active: int, quantum: int) -> dict:
     burst = [0.0] * M
     plain, plain_out = _serve_stream(kw, reqs, burst, 1, quantum)
     print(f"serving: plain burst {plain}", file=sys.stderr, flush=True)
-    served, served_out = _serve_stream(kw, reqs, burst, max_active, quantum)
+    served, served_out = _serve_stream(kw, reqs, burst, max_active, quantum, autotune=True)
     print(f"serving: served burst {served}", file=sys.stderr, flush=True)
-    identical = all(np.array_equal(a, b) for a, b in zip(plain_out, served_out))
+    # served requests run autotuned launch shapes: equal to the plain path to FP32 rounding
+    rel = max(float(np.linalg.norm(b.astype(np.float64) - a) / max(np.linalg.norm(a.astype(np.float64)), 1e-30))
+              for a, b in zip(plain_out, served_out))
     del plain_out, served_out
     out = {"requests": M, "sample": f"c5_request(0..{M - 1}) of the seeded serving stream",
            "updates": sum(_updates(f) for f in reqs), "steps_per_quantum": quantum,
            "burst": {"plain": plain, "served": served,
                      "speedup": round(served["updates_per_s"] / plain["updates_per_s"], 3)},
-           "identical_to_plain": bool(identical), "poisson": []}
+           "max_rel_l2_vs_plain": rel, "matches_plain": bool(rel <= 1e-5), "poisson": []}
     capacity = M / plain["makespan_s"]  # plain-path requests per second
     rng = np.random.default_rng(5999)
     gaps = rng.exponential(1.0, M)
@@ -150,7 +153,7 @@ def serving_leg(kw, M: int, rates, max_active: int, quantum: int) -> dict:
         lam = r * capacity
         arr = list(np.cumsum(gaps / lam) - gaps[0] / lam)
         p, _ = _serve_stream(kw, reqs, arr, 1, quantum)
-        s, _ = _serve_stream(kw, reqs, arr, max_active, quantum)
+        s, _ = _serve_stream(kw, reqs, arr, max_active, quantum, autotune=True)
         out["poisson"].append({"rate_per_s": round(lam, 3), "load_vs_plain_capacity": r, "plain": p, "served": s})
         print(f"serving: poisson load {r}: plain p50/p99 {p['p50_latency_s']}/{p['p99_latency_s']} s, "
               f"served {s['p50_latency_s']}/{s['p99_latency_s']} s", file=sys.stderr, flush=True)

@@ -179,6 +179,14 @@ const char* kw_last_error(void);
 kw_status kw_sim_set_profiling(kw_sim* sim, int enable);
 int kw_sim_profile_get(kw_sim* sim, const char** names, double* total_ms, uint64_t* launches, int cap);
 
+/* Launch-geometry autotuner: times each candidate set of workgroup sizes of
+ * the row, pencil and plane kernels over `trial_steps` steps of this
+ * simulation, keeps the fastest (cached per grid shape and physics branch for
+ * the rest of the process) and resets the simulation to step 0.  Call before
+ * kw_sim_run.  The chosen sizes are returned (0: kernel kind not used).
+ * No counterpart in the reference (its binaries have fixed launch shapes). */
+kw_status kw_sim_autotune(kw_sim* sim, int trial_steps, int* row_nt, int* pencil_nt, int* plane_nt);
+
 /* Device memory (bytes) a description would need; 0 on invalid input.
  * Used for HBM-footprint admission by the serving loop. */
 uint64_t kw_sim_footprint_bytes(const kw_sim_desc* desc);
@@ -209,6 +217,8 @@ typedef struct kw_server_config {
   uint64_t hbm_budget_bytes;  /* 0: 90% of the device memory free at create */
   uint32_t max_active;        /* simulations resident at once; 0: 8 */
   uint32_t steps_per_quantum; /* time steps queued per submission; 0: 16 */
+  uint32_t autotune;          /* 1: kw_sim_autotune each admitted simulation (cached per shape) */
+  uint32_t reserved;
 } kw_server_config;
 
 typedef enum kw_request_state {

@@ -117,8 +117,9 @@ bool fast_length(int n);
 
 // plane pass (kw_plane.hip): LDS bytes for a (Ny, Nz) plane, 0 if none
 size_t plane_lds_bytes(int ny, int nz);
+// nt: workgroup size 256, 512 or 1024; 0 picks the per-size default
 bool launch_plane(hipStream_t s, const Grid& g, const float2* twY, const float2* twZ, const LastArgs& a,
-                  const SpecTables& T);
+                  const SpecTables& T, int nt = 0);
 
 template <int NT>
 bool launch_ypass_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign);

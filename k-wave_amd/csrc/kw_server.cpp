@@ -98,6 +98,7 @@ struct kw_server {
   int device = 0;
   uint64_t budget = 0, reserved = 0, peak = 0;
   unsigned max_active = 8, quantum = 16;
+  bool autotune = false;
   Clock::time_point t0;
 
   std::mutex mu;
@@ -143,6 +144,12 @@ void kw_server::run_request(std::shared_ptr<Request> r) {
   kw_status st = kw_sim_create(&r->in->d, &sim);
   if (st != KW_OK) {
     finish(r, st, kw_last_error());
+    return;
+  }
+  if (autotune && (st = kw_sim_autotune(sim, 4, nullptr, nullptr, nullptr)) != KW_OK) {
+    const std::string m = kw_last_error();
+    kw_sim_destroy(sim);
+    finish(r, st, m);
     return;
   }
   // the host copies of the inputs are no longer needed
@@ -256,6 +263,7 @@ kw_status kw_server_create(const kw_server_config* cfg, kw_server** out) {
   s->device = cfg->device;
   s->max_active = cfg->max_active ? cfg->max_active : 8;
   s->quantum = cfg->steps_per_quantum ? cfg->steps_per_quantum : 16;
+  s->autotune = cfg->autotune != 0;
   if (hipSetDevice(s->device) != hipSuccess) return srv_fail(KW_ERR_DEVICE, "hipSetDevice failed");
   s->budget = cfg->hbm_budget_bytes;
   if (!s->budget) {

@@ -10,6 +10,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -112,6 +115,7 @@ struct kw_sim {
   float2 *twX = nullptr, *twY = nullptr, *twL = nullptr;
   XLaunch xl{};
   TileLaunch tly{}, tll{};
+  int plane_nt = 0;  // 0: per-size default
   // state
   float* u[3] = {};
   float* rho[3] = {};
@@ -317,7 +321,8 @@ void last(kw_sim* s, hipStream_t st, const LastArgs& a) {
 void spectral(kw_sim* s, hipStream_t st, const LastArgs& a) {
   if (s->g.kxm) {
     Prof pr(s, st, KK_PLANE);
-    if (!launch_plane(st, s->g, s->twY, s->twL, a, s->T)) throw HipError{hipErrorInvalidConfiguration, "launch_plane"};
+    if (!launch_plane(st, s->g, s->twY, s->twL, a, s->T, s->plane_nt))
+      throw HipError{hipErrorInvalidConfiguration, "launch_plane"};
     return;
   }
   ypass(s, st, a.in, a.nin, -1.f);
@@ -623,6 +628,45 @@ void ckpt_header(const kw_sim* s, uint64_t* h) {
 }
 }  // namespace
 
+
+// ---- autotuning of launch geometry: workgroup sizes of the row, pencil and
+// plane kernels, timed over a few steps per candidate; results are cached per
+// grid shape and physics branch for the process (the serving loop tunes each
+// shape once).
+namespace {
+struct TuneKey {
+  int Nx, Ny, Nz, kxm, absorbing, nonlinear, psrc, usrc;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(Nx, Ny, Nz, kxm, absorbing, nonlinear, psrc, usrc) <
+           std::tie(o.Nx, o.Ny, o.Nz, o.kxm, o.absorbing, o.nonlinear, o.psrc, o.usrc);
+  }
+};
+struct TuneCfg {
+  int row, pencil, plane;
+  double ms;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneCfg> g_tune_cache;
+
+TuneKey tune_key(const kw_sim* s) {
+  return TuneKey{s->g.Nx,        s->g.Ny, s->g.Nz, s->g.kxm, s->m.absorbing, s->m.nonlinear,
+                 (int)(s->psrc.flag > 0), (int)(s->usrc[0].flag + s->usrc[1].flag + s->usrc[2].flag > 0)};
+}
+
+void apply_cfg(kw_sim* s, const TuneCfg& c) {
+  if (s->xl.nt) s->xl.nt = c.row;
+  if (s->tly.nt) s->tly.nt = c.pencil;
+  if (s->tll.nt) s->tll.nt = c.pencil;
+  s->plane_nt = c.plane;
+}
+
+// back to step 0: the fields carried between steps and every output cleared
+void reset_state(kw_sim* s) {
+  for (const StateBuf& b : state_buffers(s)) HIPCHK(hipMemsetAsync(b.p, 0, b.bytes, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->t = 0;
+}
+}  // namespace
 
 extern "C" {
 
@@ -1146,6 +1190,61 @@ kw_status kw_sim_checkpoint_load(kw_sim* s, const void* host, size_t bytes) {
       p += x.bytes;
     }
     s->t = (long long)h[5];
+  } catch (const HipError& e) {
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  }
+  return KW_OK;
+}
+
+kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_nt, int* plane_nt) {
+  if (!s) return fail(KW_ERR_INVALID, "null handle");
+  if (trial_steps < 1) trial_steps = 4;
+  try {
+    HIPCHK(hipSetDevice(s->device));
+    const TuneKey key = tune_key(s);
+    TuneCfg best{s->xl.nt, s->tly.nt ? s->tly.nt : s->tll.nt, s->plane_nt, 1e30};
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) {
+        best = it->second;
+        cached = true;
+      }
+    }
+    if (!cached) {
+      std::vector<int> rows{0};
+      if (s->xl.nt) rows = s->g.Nx <= 96 ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256};
+      std::vector<int> pencils = (!s->g.kxm && (s->tly.nt || s->tll.nt)) ? std::vector<int>{128, 256}
+                                                                         : std::vector<int>{0};
+      std::vector<int> planes{0};
+      if (s->g.kxm)
+        planes = s->g.Ny * s->g.Nz < 16384 ? std::vector<int>{256, 512, 1024} : std::vector<int>{512, 1024};
+      const long long steps = std::min<long long>(s->Nt, trial_steps + 1);
+      for (int r : rows)
+        for (int pe : pencils)
+          for (int pl : planes) {
+            TuneCfg c{r, pe, pl, 0.0};
+            apply_cfg(s, c);
+            reset_state(s);
+            step(s, s->stream, 1, false);  // first step (code objects, caches)
+            HIPCHK(hipEventRecord(s->ev0, s->stream));
+            for (long long t = 2; t <= steps; ++t) step(s, s->stream, t, false);
+            HIPCHK(hipEventRecord(s->ev1, s->stream));
+            HIPCHK(hipEventSynchronize(s->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            c.ms = ms / std::max<long long>(1, steps - 1);
+            if (c.ms < best.ms) best = c;
+          }
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune_cache[key] = best;
+    }
+    apply_cfg(s, best);
+    reset_state(s);
+    if (row_nt) *row_nt = s->xl.nt;
+    if (pencil_nt) *pencil_nt = s->tly.nt ? s->tly.nt : s->tll.nt;
+    if (plane_nt) *plane_nt = s->plane_nt;
   } catch (const HipError& e) {
     return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
   }

@@ -40,7 +40,7 @@ EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
     "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
     "kw_sim_footprint_bytes", "kw_diag_copy", "kw_sim_stream",
-    "kw_sim_checkpoint_bytes", "kw_sim_checkpoint_save", "kw_sim_checkpoint_load",
+    "kw_sim_checkpoint_bytes", "kw_sim_checkpoint_save", "kw_sim_checkpoint_load", "kw_sim_autotune",
     "kw_server_create", "kw_server_submit", "kw_server_poll", "kw_server_wait", "kw_server_request_info",
     "kw_server_request_error", "kw_server_output_bytes", "kw_server_fetch", "kw_server_release",
     "kw_server_stats_get", "kw_server_destroy",
@@ -81,7 +81,8 @@ class KwSimDesc(C.Structure):
 
 class KwServerConfig(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
-                ("max_active", C.c_uint32), ("steps_per_quantum", C.c_uint32)]
+                ("max_active", C.c_uint32), ("steps_per_quantum", C.c_uint32), ("autotune", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class KwRequestInfo(C.Structure):
@@ -133,6 +134,9 @@ def load_library(path: str = LIB_PATH):
     lib.kw_sim_footprint_bytes.restype = C.c_uint64
     lib.kw_diag_copy.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
     lib.kw_diag_copy.restype = C.c_int
+    lib.kw_sim_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]
+    lib.kw_sim_autotune.restype = C.c_int
     lib.kw_sim_stream.argtypes = [C.c_void_p]
     lib.kw_sim_stream.restype = C.c_void_p
     vp, u64 = C.c_void_p, C.c_uint64
@@ -329,6 +333,13 @@ class Simulation:
     def outputs(self) -> dict:
         return {n: self.fetch(n) for n in output_names(self.record, self.dim)}
 
+    def autotune(self, trial_steps: int = 4) -> dict:
+        """Pick the fastest workgroup sizes (cached per shape); resets to step 0."""
+        r, pe, pl = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib, self.lib.kw_sim_autotune(self.handle, int(trial_steps), C.byref(r), C.byref(pe),
+                                                  C.byref(pl)))
+        return {"row_nt": r.value, "pencil_nt": pe.value, "plane_nt": pl.value}
+
     def set_profiling(self, on: bool = True):
         _check(self.lib, self.lib.kw_sim_set_profiling(self.handle, 1 if on else 0))
 
@@ -386,10 +397,10 @@ class Server:
     by its device footprint and run concurrently with the others."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_active: int = 8,
-                 steps_per_quantum: int = 16):
+                 steps_per_quantum: int = 16, autotune: bool = False):
         self.lib = load_library()
         cfg = KwServerConfig(KW_ABI_VERSION, device, int(hbm_budget_bytes), int(max_active),
-                             int(steps_per_quantum))
+                             int(steps_per_quantum), 1 if autotune else 0, 0)
         h = C.c_void_p()
         _check(self.lib, self.lib.kw_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h
