@@ -126,6 +126,15 @@ typedef struct kw_sim_desc {
   /* outputs */
   uint64_t record_flags;       /* kw_record_flag bits */
   uint64_t record_start_index; /* the binary's -s, 1-based (0 treated as 1) */
+  /* transducer source (kspaceFirstOrder3D.m:889-902; 3-D only): while
+   * t <= transducer_source_flag, ux(u_source_index(j)) +=
+   * transducer_source_input(delay_mask(j) + t - 1) (1-based), the signal
+   * pre-scaled by 2 c0 dt / dx (scaleSourceTerms.m:376-386).  Shares
+   * u_source_index / u_source_npoints and excludes ux/uy/uz sources. */
+  uint64_t transducer_source_flag;
+  uint64_t transducer_source_length;    /* samples in transducer_source_input */
+  const float* transducer_source_input;
+  const uint64_t* delay_mask;           /* u_source_npoints, 1-based (inputChecking.m:1213) */
 } kw_sim_desc;
 
 typedef struct kw_sim kw_sim;

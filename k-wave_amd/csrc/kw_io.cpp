@@ -480,7 +480,7 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
   }
   Reader R{in};
   // flags this solver does not run (saveToDisk.m:44-64)
-  const char* unsupported[] = {"elastic_flag", "axisymmetric_flag", "nonuniform_grid_flag", "transducer_source_flag",
+  const char* unsupported[] = {"elastic_flag", "axisymmetric_flag", "nonuniform_grid_flag",
                                "sxx_source_flag", "syy_source_flag", "szz_source_flag", "sxy_source_flag",
                                "sxz_source_flag", "syz_source_flag"};
   for (const char* n : unsupported) {
@@ -544,6 +544,18 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
     if (d.ux_source_flag) d.ux_source_input = R.fptr("ux_source_input");
     if (d.uy_source_flag) d.uy_source_input = R.fptr("uy_source_input");
     if (d.uz_source_flag) d.uz_source_input = R.fptr("uz_source_input");
+  }
+  // transducer source (saveToDisk.m:290-293, 339-343): the signal, per-point
+  // delay_mask, and the active points in u_source_index
+  d.transducer_source_flag = R.u64("transducer_source_flag");
+  if (d.transducer_source_flag) {
+    uint64_t np = 0;
+    d.transducer_source_input = R.fptr("transducer_source_input", &d.transducer_source_length);
+    d.delay_mask = R.uptr("delay_mask", &np);
+    if (!d.u_source_index) d.u_source_index = R.uptr("u_source_index", &d.u_source_npoints);
+    if (!R.st && np != d.u_source_npoints)
+      R.st = iofail(KW_ERR_INVALID, "delay_mask holds %llu values for %llu source points", (unsigned long long)np,
+                    (unsigned long long)d.u_source_npoints);
   }
   // Cuboid corners (sensor_mask_type 1, saveToDisk.m:357-378): columns
   // [x1 y1 z1 x2 y2 z2] of 1-based inclusive corners.  The solver records the

@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(256) k_x_velocity(Grid g, FftAxis ax, const fl
           } else {
             const int j = slot[(2 * p + h) * Nx + x];
             if (j >= 0) {
-              const float sv = S.sig[S.many ? j : 0];
+              const float sv = S.delay ? S.sig[S.delay[j]] : S.sig[S.many ? j : 0];
               v = (S.mode == 0) ? sv : v + sv;
             }
           }

@@ -47,6 +47,7 @@ struct Src {
   const float* pattern;  // additive, single signal: corrected unit pattern (N)
   const float2* spec;    // additive, many signals: X-domain corrected source
   float sval;            // additive single signal: value at this step
+  const int* delay;      // transducer: point j takes sig[delay[j]] (sig offset to this step)
 };
 
 // Sensor recording for one quantity.

@@ -78,6 +78,8 @@ struct OwnedDesc {
     d.uy_source_input = keep_f(s.uy_source_input, s.uy_source_flag * useries);
     d.uz_source_input = keep_f(s.uz_source_input, s.uz_source_flag * useries);
     d.sensor_mask_index = keep_u(s.sensor_mask_index, s.sensor_npoints);
+    d.transducer_source_input = keep_f(s.transducer_source_input, s.transducer_source_length);
+    d.delay_mask = keep_u(s.delay_mask, s.transducer_source_flag ? s.u_source_npoints : 0);
   }
 };
 

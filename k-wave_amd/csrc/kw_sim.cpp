@@ -91,6 +91,7 @@ const char* kKindNames[KK_COUNT] = {"y_pass", "last_axis", "x_velocity", "x_dens
                                     "x_pressure", "x_init", "x_source", "yz_plane"};
 
 struct SrcData {
+  int* delay = nullptr;  // transducer: per-point 0-based delay into the signal
   uint64_t flag = 0;  // signal length
   int mode = 0, many = 0;
   uint64_t nseries = 0, npoints = 0;
@@ -356,6 +357,7 @@ Src make_src(const kw_sim* s, const SrcData& d, long long t) {
   r.lin = d.lin;
   r.pattern = d.pattern;
   r.spec = d.spec;
+  r.delay = d.delay;
   if (r.active) {
     r.sig = d.sig + (size_t)(t - 1) * d.nseries;
     r.sval = d.host_sig.empty() ? 0.f : d.host_sig[(size_t)(t - 1) * d.nseries];
@@ -704,6 +706,7 @@ uint64_t kw_sim_footprint_bytes(const kw_sim_desc* d) {
   b += src(d->ux_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
   b += src(d->uy_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
   if (dim == 3) b += src(d->uz_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
+  if (d->transducer_source_flag) b += d->transducer_source_length * 4 + d->u_source_npoints * 16;
   return b;
 }
 
@@ -760,6 +763,24 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     for (int a = 0; a < 3; ++a)
       if (uflag_chk[a] && !uin_chk[a]) return fail(KW_ERR_INVALID, "u source: input of component %d missing", a);
     if ((st = load_indices(d->u_source_index, d->u_source_npoints, N, "u_source_index", ulin))) return st;
+  }
+  std::vector<int> tdelay;  // transducer: 0-based delays
+  if (d->transducer_source_flag) {
+    if (d->Nz < 2) return fail(KW_ERR_INVALID, "transducer sources are 3-D only (inputChecking.m:1160)");
+    if (uflag_chk[0] || uflag_chk[1] || uflag_chk[2])
+      return fail(KW_ERR_INVALID, "a transducer source shares u_source_index and excludes ux/uy/uz sources");
+    if (!d->u_source_index || d->u_source_npoints == 0 || !d->delay_mask || !d->transducer_source_input)
+      return fail(KW_ERR_INVALID, "transducer source: u_source_index, delay_mask or input missing");
+    if ((st = load_indices(d->u_source_index, d->u_source_npoints, N, "u_source_index", ulin))) return st;
+    tdelay.resize(d->u_source_npoints);
+    for (uint64_t j = 0; j < d->u_source_npoints; ++j) {
+      const uint64_t dm = d->delay_mask[j];
+      if (dm < 1 || dm - 1 + d->transducer_source_flag > d->transducer_source_length)
+        return fail(KW_ERR_INVALID, "delay_mask[%llu] = %llu: signal of %llu samples too short for %llu steps",
+                    (unsigned long long)j, (unsigned long long)dm, (unsigned long long)d->transducer_source_length,
+                    (unsigned long long)d->transducer_source_flag);
+      tdelay[j] = (int)(dm - 1);
+    }
   }
   kw_sim* s = new kw_sim();
   try {
@@ -991,6 +1012,21 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
         delete s;
         return st;
       }
+    }
+    if (d->transducer_source_flag) {
+      // ux additive-no-correction with a per-point offset into one signal (3D.m:889-902)
+      std::vector<int> rows;
+      build_rows(ulin, g, rows);
+      SrcData& sd = s->usrc[0];
+      sd.flag = d->transducer_source_flag;
+      sd.mode = 1;
+      sd.many = 0;
+      sd.npoints = d->u_source_npoints;
+      sd.nseries = 1;
+      sd.row_ptr = s->upload(rows.data(), rows.size());
+      sd.lin = s->upload(ulin.data(), ulin.size());
+      sd.sig = s->upload(d->transducer_source_input, (size_t)d->transducer_source_length);
+      sd.delay = s->upload(tdelay.data(), tdelay.size());
     }
     const uint64_t uf[3] = {d->ux_source_flag, d->uy_source_flag, g.dim == 3 ? d->uz_source_flag : 0};
     const float* uin[3] = {d->ux_source_input, d->uy_source_input, d->uz_source_input};

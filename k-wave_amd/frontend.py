@@ -241,6 +241,25 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
         f["u_source_many"] = _scalar(many, np.uint64)
         f["u_source_index"] = _col(idx)
 
+    # transducer source (inputChecking.m:1160-1213, scaleSourceTerms.m:376-386,
+    # saveToDisk.m:290-293, 339-343): one input signal, a delay (in samples)
+    # per active point, x velocity only; transmit apodization is not part of
+    # the file (the binaries apply none)
+    if source.get("transducer") is not None:
+        if dim != 3:
+            raise ValueError("Transducer inputs are only compatible with 3D simulations.")
+        tr = source["transducer"]
+        sig = np.asarray(tr["input_signal"], dtype=np.float64).reshape(-1)
+        delay = np.asarray(tr["delay_mask"])
+        active = np.asarray(tr["active_elements_mask"])
+        idx = _find(active)
+        flag = sig.size - int(delay.max())
+        c_here = float(np.mean(c0.reshape(-1, order="F")[idx - 1])) if hetero_c else c0_s
+        f["transducer_source_flag"] = _scalar(flag, np.uint64)
+        f["transducer_source_input"] = _signal(sig[None] * (2 * c_here * dt / kgrid.dx))
+        f["delay_mask"] = _col(delay.reshape(-1, order="F")[idx - 1].astype(np.int64) + 1)
+        f["u_source_index"] = _col(idx)
+
     # sensor: binary mask, or cuboid corners (saveToDisk.m:35-37, 355-378)
     if sensor.get("corners") is not None:
         c = np.asarray(sensor["corners"], dtype=np.int64)

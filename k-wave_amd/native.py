@@ -76,6 +76,8 @@ class KwSimDesc(C.Structure):
         ("uz_source_input", C.POINTER(C.c_float)),
         ("sensor_npoints", C.c_uint64), ("sensor_mask_index", C.POINTER(C.c_uint64)),
         ("record_flags", C.c_uint64), ("record_start_index", C.c_uint64),
+        ("transducer_source_flag", C.c_uint64), ("transducer_source_length", C.c_uint64),
+        ("transducer_source_input", C.POINTER(C.c_float)), ("delay_mask", C.POINTER(C.c_uint64)),
     ]
 
 
@@ -252,6 +254,15 @@ def build_desc(f: dict, record=("p",), record_start_index: int = 1, device: int 
         for k in ("ux", "uy", "uz"):
             if _int(f.get(k + "_source_flag")):
                 setattr(d, k + "_source_input", ptr_f(f[k + "_source_input"]))
+    d.transducer_source_flag = _int(f.get("transducer_source_flag"))
+    if d.transducer_source_flag:
+        sig = _f32(f["transducer_source_input"])
+        d.transducer_source_length = sig.size
+        keep.append(sig)
+        d.transducer_source_input = sig.ctypes.data_as(C.POINTER(C.c_float))
+        d.delay_mask = ptr_u(f["delay_mask"])
+        d.u_source_npoints = np.asarray(f["u_source_index"]).size
+        d.u_source_index = ptr_u(f["u_source_index"])
     if _int(f.get("sensor_mask_type")) == 1:
         raise ValueError("cuboid-corner sensor masks are split per cuboid by the file path (kw_run_file / "
                          "kspaceFirstOrder-HIP); in process, pass the cuboids' points as a binary mask")

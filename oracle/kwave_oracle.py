@@ -199,6 +199,12 @@ def decode(inp: dict) -> Problem:
     p0 = None
     if _int(g("p0_source_flag", 0)):
         p0 = np.asarray(inp["p0_source_input"], dtype=np.float64).reshape(shape)
+    extras = {}
+    if _int(g("transducer_source_flag", 0)):
+        # saveToDisk.m:290-293, 339-343; delay_mask is 1-based (inputChecking.m:1213)
+        extras["transducer_flag"] = _int(inp["transducer_source_flag"])
+        extras["transducer_signal"] = np.asarray(inp["transducer_source_input"], dtype=np.float64).reshape(-1)
+        extras["transducer_delay0"] = np.asarray(inp["delay_mask"], dtype=np.int64).reshape(-1) - 1
     return Problem(
         Nx=Nx, Ny=Ny, Nz=Nz, Nt=_int(inp["Nt"]), dim=dim,
         dx=_scalar(inp["dx"]), dy=_scalar(inp["dy"]),
@@ -224,6 +230,7 @@ def decode(inp: dict) -> Problem:
         u_source_index=idx("u_source_index"),
         u_source_input=u_inputs,
         sensor_mask_index=idx("sensor_mask_index"),
+        extras=extras,
     )
 
 
@@ -409,6 +416,10 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
                     u[a] = u[a] + sm
                 else:
                     uf[u_idx] = uf[u_idx] + vals
+        # transducer source (3D.m:889-902): one signal, a per-point delay, x velocity only
+        if pb.extras.get("transducer_flag", 0) >= t_index:
+            uf = u[0].reshape(-1)
+            uf[u_idx] = uf[u_idx] + pb.extras["transducer_signal"][pb.extras["transducer_delay0"] + t_index - 1]
         # divergence (3D.m:905-907)
         du = [sp.inv_real(ops.d_neg[a] * ops.kappa * sp.fwd(u[a])) for a in range(dim)]
         # mass conservation (3D.m:910-925)

@@ -108,3 +108,30 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64)
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def case_transducer(N=(48, 40, 32), Nt=60, seed=5, het=True, steer=True, pml=4):
+    """3-D transducer source: active elements on a rectangle in the plane x = pml + 3,
+    beam-steering delays along y (0..7 samples), one tone-burst input signal."""
+    rng = np.random.default_rng(seed)
+    Nx, Ny, Nz = N
+    dx = 2e-4
+    kgrid = fe.KWaveGrid(Nx, dx, Ny, dx, Nz, dx)
+    shape = (Nx, Ny, Nz)
+    c = 1500.0 * (1 + (0.03 * smooth_field(rng, shape) if het else 0.0))
+    rho = 1000.0 * (1 + (0.03 * smooth_field(rng, shape) if het else 0.0))
+    kgrid.set_time(Nt, 0.3 * dx / float(np.max(c)))
+    active = np.zeros(shape)
+    active[pml + 2, Ny // 2 - 8: Ny // 2 + 8, Nz // 2 - 4: Nz // 2 + 4] = 1
+    delay = np.zeros(shape, dtype=np.int64)
+    if steer:
+        yy = np.arange(Ny)[None, :, None] * np.ones(shape, dtype=np.int64)
+        delay = np.where(active > 0, (yy - (Ny // 2 - 8)) // 2, 0)
+    sig = geo.tone_burst(1 / kgrid.dt, 1.5e6, 3)[0] * 0.5
+    sig = np.concatenate([sig, np.zeros(Nt)])
+    medium = {"sound_speed": c, "density": rho, "alpha_coeff": 0.5, "alpha_power": 1.5}
+    mask = np.zeros(shape)
+    mask[:, :, Nz // 2] = 1
+    return fe.save_to_disk_dict(kgrid, medium, {"transducer": {"input_signal": sig, "delay_mask": delay,
+                                                                "active_elements_mask": active}},
+                                {"mask": mask}, pml_size=pml)

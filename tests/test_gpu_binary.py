@@ -210,3 +210,20 @@ def test_checkpoint_restart_is_exact(kw, tmp_path, how):
     assert sorted(a) == sorted(b)
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_transducer_through_the_file(kw, tmp_path):
+    """transducer_source_flag / transducer_source_input / delay_mask read from the
+    input file give the in-process result bit for bit."""
+    from kwave_amd import h5format
+
+    print("[gpu binary] transducer source via kspaceFirstOrder-HIP", flush=True)
+    f = cases.case_transducer(Nt=40)
+    inp, out = str(tmp_path / "in.h5"), str(tmp_path / "out.h5")
+    h5format.write_input(inp, f)
+    r = h5format.run_binary(inp, out, ("p", "u_final"))
+    assert r.returncode == 0, r.stderr
+    o = h5format.read_file(out)
+    direct = kw.run(f, record=("p", "u_final"))
+    for k in direct:
+        np.testing.assert_array_equal(o[k].reshape(direct[k].shape), direct[k])

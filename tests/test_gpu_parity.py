@@ -196,3 +196,10 @@ def test_u_non_staggered(kw, oracle, case):
         f = cases.case_3d(N=(32, 32, 48), Nt=20, pml=4, p0=True, source=None)
         rec = ("u_non_staggered", "u")
     compare(kw, oracle, f, rec, f"u_non_staggered_{case}", start=3 if case == "2d" else 1)
+
+
+@pytest.mark.parametrize("steer", [True, False])
+def test_transducer_source(kw, oracle, steer):
+    """kWaveTransducer-style source (3D.m:889-902) with and without steering delays."""
+    f = cases.case_transducer(steer=steer)
+    compare(kw, oracle, f, ("p", "p_max", "u_final"), f"transducer_steer{steer}")

@@ -412,3 +412,26 @@ def test_u_non_staggered_is_half_cell_shift(oracle):
     k = oracle.k_vec_unshifted(n, 1.0)
     ush = np.real(np.fft.ifft(np.exp(-1j * k / 2) * np.fft.fft(u)))
     assert np.allclose(ush, np.cos(2 * np.pi * 5 * x / n + 0.3), atol=1e-12)
+
+
+def test_transducer_equals_delayed_many_source(oracle):
+    """3D.m:889-902: point j adds signal(delay_mask(j) + t - 1) to ux.  The same
+    field follows from a `many` additive-no-correction ux source whose series j is
+    the signal advanced by delay_j (a one-sample or sign mistake in the delay
+    indexing breaks the equality)."""
+    import cases
+    f = cases.case_transducer(N=(32, 28, 24), Nt=30)
+    a = oracle.run(f, record=("p", "u_final"))
+    flag = int(np.ravel(f["transducer_source_flag"])[0])
+    sig = np.asarray(f["transducer_source_input"], dtype=np.float64).reshape(-1)
+    d0 = np.asarray(f["delay_mask"], dtype=np.int64).reshape(-1) - 1
+    series = np.stack([sig[d + np.arange(flag)] for d in d0])          # (Npts, flag)
+    g = {k: v for k, v in f.items() if k not in ("transducer_source_flag", "transducer_source_input", "delay_mask")}
+    g["ux_source_flag"] = np.array([[[flag]]], dtype=np.uint64)
+    g["u_source_mode"] = np.array([[[1]]], dtype=np.uint64)
+    g["u_source_many"] = np.array([[[1]]], dtype=np.uint64)
+    g["ux_source_input"] = np.ascontiguousarray(series.T[None].astype(np.float32))
+    b = oracle.run(g, record=("p", "u_final"))
+    for k in a:
+        assert np.allclose(a[k], b[k], rtol=0, atol=1e-12 * np.abs(b[k]).max()), k
+    assert flag == sig.size - 7
