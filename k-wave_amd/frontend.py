@@ -92,6 +92,27 @@ def _signal(sig: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(s.T[None].astype(np.float32))
 
 
+def _source_labels(mask: np.ndarray, nrows: int, sig_name: str, mask_name: str):
+    """Label of every source point, in find() order, for a labelled source
+    mask; None for a binary mask (kspaceFirstOrder_inputChecking.m:847-884,
+    964-1003).  The in-process solver then takes the signal row of each
+    point's label (kspaceFirstOrder3D.m:933-957); the input file has no label
+    dataset (p_source_many only says "one row per point"), so the front end
+    expands the rows to one per point."""
+    m = np.asarray(mask)
+    u = np.unique(m)
+    if u.size <= 2 and u.sum() == 1:
+        return None
+    if np.sum(u[1:] - u[:-1]) != u.size - 1 or not np.any(u == 1):
+        raise ValueError(f"If using a labelled source.{mask_name}, the source labels must be monotonically "
+                         "increasing and start from 1.")
+    if nrows != u.size - 1:
+        raise ValueError(f"The number of time series in source.{sig_name} must match the number of labelled "
+                         f"source elements in source.{mask_name}.")
+    v = m.reshape(-1, order="F")
+    return v[v != 0].astype(np.int64)
+
+
 def _stagger_c(rc: np.ndarray, axis: int) -> np.ndarray:
     """interpn('*linear') at +d/2 along a MATLAB axis (0 = x), last plane copied
     (3D.m:583-596), on a C-order (Nz, Ny, Nx) float64 array; returns float32."""
@@ -113,7 +134,7 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
     """Dataset dict of the input file (kspaceFirstOrder_saveToDisk.m:44-414).
 
     medium: sound_speed, density, [alpha_coeff, alpha_power, alpha_mode='stokes'], [BonA]
-    source: [p0] | [p, p_mask, p_mode] | [ux/uy/uz, u_mask, u_mode]
+    source: [p0] | [p, p_mask, p_mode] | [ux/uy/uz, u_mask, u_mode]; masks binary or labelled 1..L
     sensor: mask (binary, MATLAB order)
     """
     dim = kgrid.dim
@@ -213,6 +234,12 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
                 sig[r] = sig[r] / (dim * c_here ** 2)
             else:
                 sig[r] = sig[r] * (2 * dt / (dim * c_here * kgrid.dx))
+        # labelled mask: rows were scaled with the sound speed at the r-th
+        # source point, as scaleSourceTerms.m:60-62,120-122 index them, then
+        # each point takes its label's row
+        labels = _source_labels(source["p_mask"], sig.shape[0], "p", "p_mask")
+        if labels is not None:
+            sig = sig[labels - 1]
         f["p_source_flag"] = _scalar(sig.shape[1], np.uint64)
         f["p_source_mode"] = _scalar(mode_code[mode], np.uint64)
         f["p_source_many"] = _scalar(int(sig.shape[0] > 1), np.uint64)
@@ -224,6 +251,9 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
     if u_keys:
         idx = _find(source["u_mask"])
         mode = source.get("u_mode", "additive")
+        labels = None
+        for k in u_keys:
+            labels = _source_labels(source["u_mask"], np.atleast_2d(np.asarray(source[k])).shape[0], k, "u_mask")
         many = None
         for a, k in enumerate(("ux", "uy", "uz")[:dim]):
             if source.get(k) is None:
@@ -233,6 +263,8 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
                 for r in range(sig.shape[0]):
                     c_here = c_at(idx[r]) if hetero_c else c0_s
                     sig[r] = sig[r] * (2 * c_here * dt / d[a])
+            if labels is not None:
+                sig = sig[labels - 1]
             if many is None:
                 many = int(sig.shape[0] > 1)
             f[k + "_source_flag"] = _scalar(sig.shape[1], np.uint64)

@@ -135,3 +135,46 @@ def case_transducer(N=(48, 40, 32), Nt=60, seed=5, het=True, steer=True, pml=4):
     return fe.save_to_disk_dict(kgrid, medium, {"transducer": {"input_signal": sig, "delay_mask": delay,
                                                                 "active_elements_mask": active}},
                                 {"mask": mask}, pml_size=pml)
+
+
+def case_labelled(kind="p", mode="additive", het=False, only=None, Nt=60, seed=7):
+    """Labelled source mask (kspaceFirstOrder_inputChecking.m:847-884, 964-1003):
+    labels 1..3 scattered over a line of source points, one signal row per
+    label.  kind "p": 2-D 48x40 pressure source; kind "u": 3-D 28x24x20 ux/uy
+    source.  only=k keeps just label k's points as a binary mask driven by
+    row k alone (the superposition pieces)."""
+    rng = np.random.default_rng(seed)
+    dim = 2 if kind == "p" else 3
+    shape = (48, 40) if dim == 2 else (28, 24, 20)
+    dx = 1e-4
+    kgrid = fe.KWaveGrid(shape[0], dx, shape[1], dx) if dim == 2 else fe.KWaveGrid(*[v for n in shape for v in (n, dx)])
+    c = 1500.0 * (1 + (0.03 * smooth_field(rng, shape) if het else 0.0))
+    rho = 1000.0 * (1 + (0.03 * smooth_field(rng, shape) if het else 0.0))
+    kgrid.set_time(Nt, 0.3 * dx / float(np.max(c)))
+    m = np.zeros(shape)
+    line = (slice(10, 11), slice(6, 34)) if dim == 2 else (slice(8, 9), slice(5, 19), slice(9, 11))
+    npts = int(np.prod([s.stop - s.start for s in line]))
+    labels = 1 + (np.arange(npts) * 5 + np.arange(npts) // 3) % 3
+    m[line] = labels.reshape([s.stop - s.start for s in line], order="F")
+    m[tuple(min(n - 6, 30) for n in shape)] = 2                          # a stray point of label 2
+    t = np.arange(Nt) * kgrid.dt
+    rows = np.stack([np.sin(2 * np.pi * f0 * t) * np.exp(-((t - t0) / 4e-7) ** 2)
+                     for f0, t0 in ((1.0e6, 8e-7), (1.6e6, 1.1e-6), (0.7e6, 1.4e-6))])
+    if only is not None:
+        m = (m == only).astype(float)
+        rows = rows[only - 1:only]
+    src = ({"p": rows * 1e5, "p_mask": m, "p_mode": mode} if kind == "p"
+           else {"ux": rows * 0.5, "uy": rows[::-1] * 0.3 if only is None else rows * 0.3,
+                 "u_mask": m, "u_mode": mode})
+    if kind == "u" and only is not None:
+        # the uy rows are reversed in the labelled run: label k drives uy with row 4-k
+        full = np.stack([np.sin(2 * np.pi * f0 * t) * np.exp(-((t - t0) / 4e-7) ** 2)
+                         for f0, t0 in ((1.0e6, 8e-7), (1.6e6, 1.1e-6), (0.7e6, 1.4e-6))])
+        src["uy"] = full[::-1][only - 1:only] * 0.3
+    medium = {"sound_speed": c, "density": rho}
+    sensor = np.zeros(shape)
+    if dim == 2:
+        sensor[30, :] = 1
+    else:
+        sensor[:, :, 12] = 1
+    return fe.save_to_disk_dict(kgrid, medium, src, {"mask": sensor}, pml_size=5)

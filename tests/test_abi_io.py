@@ -254,3 +254,33 @@ def test_cuboid_corners_front_end_layout(kw):
     assert f["sensor_mask_corners"][0].tolist() == [[3, 4, 1, 8, 9, 1], [10, 2, 1, 20, 7, 1]]
     with pytest.raises(ValueError):
         kw.native.build_desc(f)
+
+
+def test_labelled_source_front_end_expansion(kw):
+    """inputChecking.m:847-884: a labelled p_mask becomes one row per source
+    point (p_source_many = 1) taking its label's row, rows scaled first with the
+    sound speed at the r-th source point (scaleSourceTerms.m:60-62); labels
+    must run 1..L with one row each."""
+    fe = kw.frontend
+    kg = fe.KWaveGrid(24, 1e-4, 20, 1e-4)
+    kg.set_time(8, 2e-8)
+    c = np.full((24, 20), 1500.0)
+    c[5, 3:9] = [1400, 1450, 1500, 1550, 1600, 1650]
+    m = np.zeros((24, 20))
+    m[5, 3:9] = [2, 1, 2, 2, 1, 2]
+    sig = np.arange(16, dtype=float).reshape(2, 8) + 1
+    f = fe.save_to_disk_dict(kg, {"sound_speed": c, "density": 1000.0},
+                             {"p": sig, "p_mask": m, "p_mode": "additive-no-correction"}, {"mask": m > 0}, pml_size=4)
+    assert int(f["p_source_many"].ravel()[0]) == 1
+    rows = f["p_source_input"][0].T                     # (Npts, Nt_src)
+    scale = 2 * 2e-8 / (2 * 1e-4)
+    expect = [sig[L - 1] * scale / c[5, 3 + (L - 1)] for L in (2, 1, 2, 2, 1, 2)]
+    assert np.allclose(rows, np.array(expect), rtol=1e-6)
+    bad = m.copy()
+    bad[5, 4] = 3                                        # labels 2, 3: no label 1
+    with pytest.raises(ValueError, match="start from 1"):
+        fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0},
+                             {"p": sig, "p_mask": bad * (bad != 1)}, {"mask": m > 0}, pml_size=4)
+    with pytest.raises(ValueError, match="number of time series"):
+        fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0},
+                             {"p": sig[:1].repeat(3, 0), "p_mask": m}, {"mask": m > 0}, pml_size=4)

@@ -203,3 +203,10 @@ def test_transducer_source(kw, oracle, steer):
     """kWaveTransducer-style source (3D.m:889-902) with and without steering delays."""
     f = cases.case_transducer(steer=steer)
     compare(kw, oracle, f, ("p", "p_max", "u_final"), f"transducer_steer{steer}")
+
+
+@pytest.mark.parametrize("kind,mode", [("p", "additive"), ("u", "additive"), ("p", "dirichlet")])
+def test_labelled_source_mask(kw, oracle, kind, mode):
+    """Labelled p_mask / u_mask (inputChecking.m:847-884, 964-1003) in a heterogeneous medium."""
+    f = cases.case_labelled(kind, mode=mode, het=True)
+    compare(kw, oracle, f, ("p", "p_max", "u_final"), f"labelled_{kind}_{mode}")

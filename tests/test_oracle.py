@@ -435,3 +435,21 @@ def test_transducer_equals_delayed_many_source(oracle):
     for k in a:
         assert np.allclose(a[k], b[k], rtol=0, atol=1e-12 * np.abs(b[k]).max()), k
     assert flag == sig.size - 7
+
+
+@pytest.mark.parametrize("kind,mode", [("p", "additive"), ("u", "additive-no-correction")])
+def test_labelled_source_is_sum_of_its_labels(oracle, kind, mode):
+    """A labelled source mask drives each point with its label's signal row
+    (kspaceFirstOrder3D.m:933-957 with p_source_sig_index, inputChecking.m:881-883).
+    In a homogeneous lossless medium the field is linear in the sources, so the
+    labelled run equals the sum of runs that each keep one label's points (a
+    binary mask) driven by that label's row; a label/row mix-up or an
+    off-by-one in the expansion breaks the sum."""
+    import cases
+    rec = ("p", "u_final")
+    a = oracle.run(cases.case_labelled(kind, mode=mode), record=rec)
+    parts = [oracle.run(cases.case_labelled(kind, mode=mode, only=k), record=rec) for k in (1, 2, 3)]
+    for key in a:
+        s = sum(p[key] for p in parts)
+        assert np.abs(s).max() > 0
+        assert np.allclose(a[key], s, rtol=0, atol=1e-11 * np.abs(s).max()), key
