@@ -130,8 +130,13 @@ def _stagger_c(rc: np.ndarray, axis: int) -> np.ndarray:
 
 def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict,
                       pml_size=None, pml_alpha=2.0, smooth_p0: bool = True,
-                      sound_speed_ref=None) -> dict:
+                      sound_speed_ref=None, axisymmetric: bool = False) -> dict:
     """Dataset dict of the input file (kspaceFirstOrder_saveToDisk.m:44-414).
+
+    axisymmetric: kspaceFirstOrderASC's file (axisymmetric_flag = 1): a 2-D grid
+    with x axial and y radial from the axis (y index 0 on the axis), a radial PML
+    at the outer edge only, stokes absorption (inputChecking.m:165-200) and p0
+    smoothed on the WSWA-mirrored grid (inputChecking.m:1505-1536).
 
     medium: sound_speed, density, [alpha_coeff, alpha_power, alpha_mode='stokes'], [BonA]
     source: [p0] | [p, p_mask, p_mode] | [ux/uy/uz, u_mask, u_mode]; masks binary or labelled 1..L
@@ -145,8 +150,11 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
         pml_size = 20 if dim == 2 else 10                       # setDefaults.m:149-168
     ps = list(pml_size) if np.ndim(pml_size) else [int(pml_size)] * dim
     pa = list(pml_alpha) if np.ndim(pml_alpha) else [float(pml_alpha)] * dim
+    if axisymmetric and dim != 2:
+        raise ValueError("axisymmetric simulations use a 2-D grid (x axial, y radial)")
     for a in range(dim):
-        if 2 * ps[a] >= shape[a]:
+        # inputChecking.m:1378-1379: the radial PML of the axisymmetric code is one-sided
+        if (ps[a] > shape[a]) if (axisymmetric and a == 1) else (2 * ps[a] > shape[a]):
             raise ValueError("The size of the PML must be smaller than the size of the grid.")
 
     c0 = np.asarray(medium["sound_speed"], dtype=np.float64)
@@ -208,16 +216,27 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
         f["BonA"] = _c_order(b).astype(np.float32) if b.size > 1 else _scalar(float(b.reshape(-1)[0]))
     # absorption (alpha_coeff stays in dB/(MHz^y cm): inputChecking.m:1666-1671)
     if "alpha_coeff" in medium and medium["alpha_coeff"] is not None:
-        stokes = medium.get("alpha_mode") == "stokes"
+        stokes = medium.get("alpha_mode") == "stokes" or axisymmetric
         f["absorbing_flag"] = _scalar(2 if stokes else 1, np.uint64)
         ac = np.asarray(medium["alpha_coeff"], dtype=np.float64)
         f["alpha_coeff"] = _c_order(ac).astype(np.float32) if ac.size > 1 else _scalar(float(ac.reshape(-1)[0]))
-        f["alpha_power"] = _scalar(float(medium.get("alpha_power", 2.0 if stokes else 1.5)))
+        f["alpha_power"] = _scalar(2.0 if axisymmetric else float(medium.get("alpha_power", 2.0 if stokes else 1.5)))
+    if axisymmetric:
+        f["axisymmetric_flag"] = _scalar(1, np.uint64)
 
     # initial pressure (smoothed before expansion: inputChecking.m:1499-1553)
     if source.get("p0") is not None:
         p0 = np.asarray(source["p0"], dtype=np.float64)
-        if smooth_p0:
+        if smooth_p0 and axisymmetric:
+            # WSWA mirror to 4 Ny, smooth, trim (inputChecking.m:1505-1534)
+            Ny = p0.shape[1]
+            e = np.zeros((p0.shape[0], 4 * Ny))
+            e[:, :Ny] = p0
+            e[:, Ny + 1:2 * Ny] = -p0[:, :0:-1]
+            e[:, 2 * Ny:3 * Ny] = -p0
+            e[:, 3 * Ny + 1:] = p0[:, :0:-1]
+            p0 = geometry.smooth(e, True)[:, :Ny]
+        elif smooth_p0:
             p0 = geometry.smooth(p0, True)
         f["p0_source_flag"] = _scalar(1, np.uint64)
         f["p0_source_input"] = _c_order(p0).astype(np.float32)

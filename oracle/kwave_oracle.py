@@ -19,6 +19,7 @@ file written by ``kspaceFirstOrder_saveToDisk.m``:
 * time loop                  k-Wave/kspaceFirstOrder3D.m:767-1206,
                              k-Wave/kspaceFirstOrder2D.m:710-1042
 * sensor recording           k-Wave/private/kspaceFirstOrder_extractSensorData.m:61-297, 462-560
+* axisymmetric loop          k-Wave/kspaceFirstOrderAS.m:467-571, 692-1244 (WSWA-FFT form)
 
 Input: a dict holding the datasets of the HDF5 input file, each as a numpy
 array in HDF5 C order (MATLAB [Nx,Ny,Nz] == C (Nz,Ny,Nx)); see
@@ -86,8 +87,9 @@ def sinc(x):
 
 
 def get_pml(n: int, d: float, dt: float, c: float, size: int, alpha: float,
-            staggered: bool) -> np.ndarray:
-    """1-D PML multiplier.  k-Wave/private/getPML.m:50-85."""
+            staggered: bool, axisymmetric: bool = False) -> np.ndarray:
+    """1-D PML multiplier.  k-Wave/private/getPML.m:50-85; axisymmetric: the
+    radial axis has the layer at its outer end only (getPML.m:73-77)."""
     x = np.arange(1, size + 1, dtype=np.float64)
     if staggered:
         left = alpha * (c / d) * (((x + 0.5) - size - 1) / (0 - size)) ** 4
@@ -99,7 +101,8 @@ def get_pml(n: int, d: float, dt: float, c: float, size: int, alpha: float,
     right = np.exp(-right * dt / 2)
     pml = np.ones(n, dtype=np.float64)
     if size > 0:
-        pml[:size] = left
+        if not axisymmetric:
+            pml[:size] = left
         pml[n - size:] = right
     return pml
 
@@ -158,6 +161,7 @@ class Problem:
     u_source_input: list
     sensor_mask_index: np.ndarray
     extras: dict = field(default_factory=dict)
+    axisymmetric: bool = False
 
     @property
     def shape(self):
@@ -231,6 +235,7 @@ def decode(inp: dict) -> Problem:
         u_source_input=u_inputs,
         sensor_mask_index=idx("sensor_mask_index"),
         extras=extras,
+        axisymmetric=bool(_int(g("axisymmetric_flag", 0))),
     )
 
 
@@ -269,8 +274,9 @@ class Operators:
         self.pml = []
         self.pml_sg = []
         for a in range(pb.dim):
-            self.pml.append(get_pml(n[a], d[a], pb.dt, pb.c_ref, pb.pml_size[a], pb.pml_alpha[a], False))
-            self.pml_sg.append(get_pml(n[a], d[a], pb.dt, pb.c_ref, pb.pml_size[a], pb.pml_alpha[a], True))
+            axi = pb.axisymmetric and a == 1      # radial axis (kspaceFirstOrderAS.m:470-471)
+            self.pml.append(get_pml(n[a], d[a], pb.dt, pb.c_ref, pb.pml_size[a], pb.pml_alpha[a], False, axi))
+            self.pml_sg.append(get_pml(n[a], d[a], pb.dt, pb.c_ref, pb.pml_size[a], pb.pml_alpha[a], True, axi))
         # absorption (createAbsorptionVariables.m:27-53, 83-97)
         self.absorb_tau = 0.0
         self.absorb_eta = 0.0
@@ -323,6 +329,79 @@ class Spectral:
         return sfft.irfftn(F, s=sh, axes=self.axes[::-1], workers=WORKERS)
 
 
+class AxisymOps:
+    """Axisymmetric (x axial, y radial) gradients of kspaceFirstOrderAS.m in
+    the WSWA-FFT form: each field is mirrored to a 4 Ny radial period with the
+    symmetry of its grid, transformed with fft2 on the expanded grid
+    kWaveGrid(Nx, dx, 4 Ny, dy), and trimmed back to Ny (AS.m:488-521 operators,
+    709-731 pressure gradient, 950-990 divergence, 801-827 / 876-904 / 1076-1090
+    source corrections).  The input file selects WSWA (DTT) symmetry
+    (inputChecking.m:1400-1411), which the reference's own test
+    (testing/unit/kspaceFirstOrderAS_compare_FFT_implementations.m) holds equal
+    to WSWA-FFT to 1e-14; tests/test_oracle.py repeats that comparison with
+    scipy's DCT/DST.  Expanded arrays are zero-filled for every use (AS.m keeps
+    one source_mat_exp across components, which WSWA does not have)."""
+
+    def __init__(self, pb: Problem):
+        if pb.dim != 2:
+            raise ValueError("axisymmetric inputs are 2-D (x axial, y radial)")
+        if pb.absorbing_flag == 1:
+            raise ValueError("the axisymmetric code supports stokes absorption only (inputChecking.m:172-185)")
+        self.Ny = pb.Ny
+        kx = k_vec_unshifted(pb.Nx, pb.dx)[None, :]
+        ky = k_vec_unshifted(4 * pb.Ny, pb.dy)[:, None]
+        self.ddx_pos = 1j * kx * np.exp(1j * kx * pb.dx / 2)       # AS.m:476-477
+        self.ddx_neg = 1j * kx * np.exp(-1j * kx * pb.dx / 2)
+        self.ddy = 1j * ky                                            # AS.m:508-510
+        self.y_pos = np.exp(1j * ky * pb.dy / 2)
+        self.y_neg = np.exp(-1j * ky * pb.dy / 2)
+        k = np.sqrt(kx ** 2 + ky ** 2)
+        self.kappa = sinc(pb.c_ref * k * pb.dt / 2)                  # AS.m:514
+        self.source_kappa = np.cos(pb.c_ref * k * pb.dt / 2)         # AS.m:516
+        self.y_sg = (np.arange(pb.Ny) + 0.5) * pb.dy                 # y_vec_sg, AS.m:571
+
+    def mirror(self, f, kind):
+        """Radial mirror to 4 Ny.  "wswa": p, ux and their sources (AS.m:713-717);
+        "hahs": uy and its source (AS.m:953-956, 884-888); "hsha": uy / y_sg (AS.m:959-962)."""
+        Ny = self.Ny
+        e = np.zeros(f.shape[:-2] + (4 * Ny, f.shape[-1]))
+        if kind == "wswa":
+            e[..., :Ny, :] = f
+            e[..., Ny + 1:2 * Ny, :] = -f[..., :0:-1, :]
+            e[..., 2 * Ny:3 * Ny, :] = -f
+            e[..., 3 * Ny + 1:, :] = f[..., :0:-1, :]
+        else:
+            sgn = 1.0 if kind == "hahs" else -1.0
+            e[..., :Ny, :] = f
+            e[..., Ny:2 * Ny, :] = sgn * f[..., ::-1, :]
+            e[..., 2 * Ny:3 * Ny, :] = -f
+            e[..., 3 * Ny:, :] = -sgn * f[..., ::-1, :]
+        return e
+
+    @staticmethod
+    def _f2(f):
+        return sfft.fft2(f, axes=(-2, -1), workers=WORKERS)
+
+    def _i2(self, F):
+        return np.real(sfft.ifft2(F, axes=(-2, -1), workers=WORKERS))[..., :self.Ny, :]
+
+    def grad_p(self, p):
+        """[dpdx_sgx, dpdy_sgy] (AS.m:724-731)."""
+        pk = self.kappa * self._f2(self.mirror(p, "wswa"))
+        return [self._i2(self.ddx_pos * pk), self._i2(self.ddy * self.y_pos * pk)]
+
+    def div(self, ux, uy):
+        """[duxdx, duydy]: the radial term is (1/r) d(r uy)/dr = duy/dy + uy/y (AS.m:981-986)."""
+        duxdx = self._i2(self.ddx_neg * self.kappa * self._f2(self.mirror(ux, "wswa")))
+        duydy = self._i2(self.kappa * self.y_neg * (self.ddy * self._f2(self.mirror(uy, "hahs"))
+                                                     + self._f2(self.mirror(uy / self.y_sg[:, None], "hsha"))))
+        return [duxdx, duydy]
+
+    def source_correction(self, sm, kind):
+        """real(ifft2(source_kappa .* fft2(mirror(source_mat)))) trimmed (AS.m:806-827)."""
+        return self._i2(self.source_kappa * self._f2(self.mirror(sm, kind)))
+
+
 # ---------------------------------------------------------------------------
 # time loop
 # ---------------------------------------------------------------------------
@@ -343,6 +422,7 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
     pb = decode(inp)
     ops = Operators(pb, full_complex)
     sp = Spectral(pb, full_complex)
+    axs = AxisymOps(pb) if pb.axisymmetric else None
     dim, dt = pb.dim, pb.dt
     shape = pb.shape
     Nt = pb.Nt if nsteps is None else nsteps
@@ -381,6 +461,8 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
     if "u" in rec:
         for a, nm in enumerate("xyz"[:dim]):
             out["u" + nm] = np.zeros((max(nrec, 0), ns))
+    if "u_non_staggered" in rec and axs:
+        raise ValueError("u_non_staggered is not defined for axisymmetric inputs here")
     if "u_non_staggered" in rec:
         # record.x_shift_neg = ifftshift(exp(-1i kx_vec dx / 2))
         # (kspaceFirstOrder_createStorageVariables.m:87-99); array axis of MATLAB axis a is 2 - a
@@ -398,9 +480,10 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
     u_rms_acc = [np.zeros(ns) for _ in range(dim)]
 
     for t_index in range(1, Nt + 1):
-        # velocity update (3D.m:790-801)
+        # velocity update (3D.m:790-801; AS.m:777-784)
+        grads = axs.grad_p(p) if axs else None
         for a in range(dim):
-            grad = sp.inv_real(ops.d_pos[a] * ops.kappa * p_k)
+            grad = grads[a] if axs else sp.inv_real(ops.d_pos[a] * ops.kappa * p_k)
             u[a] = pml_sg[a] * (pml_sg[a] * u[a] - dt * rsg_inv[a] * grad)
         # velocity sources (3D.m:804-884)
         for a in range(dim):
@@ -412,7 +495,10 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
                 elif pb.u_source_mode == 2:
                     sm = np.zeros(pb.N)
                     sm[u_idx] = vals
-                    sm = sp.inv_real(ops.source_kappa * sp.fwd(sm.reshape(shape)))
+                    if axs:
+                        sm = axs.source_correction(sm.reshape(shape), "wswa" if a == 0 else "hahs")
+                    else:
+                        sm = sp.inv_real(ops.source_kappa * sp.fwd(sm.reshape(shape)))
                     u[a] = u[a] + sm
                 else:
                     uf[u_idx] = uf[u_idx] + vals
@@ -421,7 +507,8 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
             uf = u[0].reshape(-1)
             uf[u_idx] = uf[u_idx] + pb.extras["transducer_signal"][pb.extras["transducer_delay0"] + t_index - 1]
         # divergence (3D.m:905-907)
-        du = [sp.inv_real(ops.d_neg[a] * ops.kappa * sp.fwd(u[a])) for a in range(dim)]
+        du = axs.div(u[0], u[1]) if axs else [sp.inv_real(ops.d_neg[a] * ops.kappa * sp.fwd(u[a]))
+                                              for a in range(dim)]
         # mass conservation (3D.m:910-925)
         if not pb.nonlinear:
             for a in range(dim):
@@ -439,7 +526,10 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
             elif pb.p_source_mode == 2:
                 sm = np.zeros(pb.N)
                 sm[p_idx] = vals
-                sm = sp.inv_real(ops.source_kappa * sp.fwd(sm.reshape(shape)))
+                if axs:
+                    sm = axs.source_correction(sm.reshape(shape), "wswa")
+                else:
+                    sm = sp.inv_real(ops.source_kappa * sp.fwd(sm.reshape(shape)))
                 for a in range(dim):
                     rho[a] = rho[a] + sm
             else:
@@ -467,8 +557,9 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
             for a in range(dim):
                 rho[a] = pb.p0 / (dim * c0 ** 2) * np.ones(shape)
             p0k = sp.fwd(p)
+            g0 = axs.grad_p(p) if axs else None
             for a in range(dim):
-                u[a] = dt * rsg_inv[a] * sp.inv_real(ops.d_pos[a] * ops.kappa * p0k) / 2
+                u[a] = dt * rsg_inv[a] * (g0[a] if axs else sp.inv_real(ops.d_pos[a] * ops.kappa * p0k)) / 2
         # spectrum of p (3D.m:1038)
         p_k = sp.fwd(p)
         # sensor recording (extractSensorData.m:61-297)

@@ -178,3 +178,40 @@ def case_labelled(kind="p", mode="additive", het=False, only=None, Nt=60, seed=7
     else:
         sensor[:, :, 12] = 1
     return fe.save_to_disk_dict(kgrid, medium, src, {"mask": sensor}, pml_size=5)
+
+
+def case_axisym(Nx=64, Ny=32, Nt=50, dx=1e-4, source="p0", het=False, absorbing=False, nonlinear=False,
+                mode="additive", sigma=2.5, pml=8, seed=11):
+    """Axisymmetric input (kspaceFirstOrderASC): x axial, y radial from the axis.
+    source "p0": a Gaussian on the axis; "p": a disc of p points across the
+    axis (a piston face); "u": ux and uy on a ring; sensors on the axis and on
+    a radial line."""
+    rng = np.random.default_rng(seed)
+    kg = fe.KWaveGrid(Nx, dx, Ny, dx)
+    c = 1500.0 * (1 + (0.03 * smooth_field(rng, (Nx, Ny)) if het else 0.0))
+    rho = 1000.0 * (1 + (0.03 * smooth_field(rng, (Nx, Ny)) if het else 0.0))
+    kg.set_time(Nt, 0.3 * dx / float(np.max(c)))
+    medium = {"sound_speed": c, "density": rho}
+    if absorbing:
+        medium["alpha_coeff"] = (0.5 + 0.3 * rng.random((Nx, Ny))) if het else 0.75
+    if nonlinear:
+        medium["BonA"] = 5.0
+    src = {}
+    if source == "p0":
+        x = (np.arange(Nx) - Nx // 2)[:, None] * dx
+        r = np.arange(Ny)[None, :] * dx
+        src["p0"] = 2.0 * np.exp(-(x ** 2 + r ** 2) / (2 * (sigma * dx) ** 2))
+    else:
+        burst = geo.tone_burst(1 / kg.dt, 1.5e6, 3)[0]
+        m = np.zeros((Nx, Ny))
+        if source == "p":
+            m[pml + 2, :Ny // 3] = 1
+            src.update({"p": burst[None] * 1e5, "p_mask": m, "p_mode": mode})
+        else:
+            m[pml + 4, Ny // 4: Ny // 4 + 3] = 1
+            m[pml + 5: pml + 8, Ny // 4] = 1
+            src.update({"ux": burst[None] * 0.5, "uy": np.roll(burst, 5)[None] * 0.3, "u_mask": m, "u_mode": mode})
+    mask = np.zeros((Nx, Ny))
+    mask[:, 0] = 1
+    mask[Nx // 2, :] = 1
+    return fe.save_to_disk_dict(kg, medium, src, {"mask": mask}, pml_size=pml, smooth_p0=False, axisymmetric=True)
