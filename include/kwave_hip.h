@@ -135,6 +135,12 @@ typedef struct kw_sim_desc {
   uint64_t transducer_source_length;    /* samples in transducer_source_input */
   const float* transducer_source_input;
   const uint64_t* delay_mask;           /* u_source_npoints, 1-based (inputChecking.m:1213) */
+  /* axisymmetric_flag (kspaceFirstOrderASC.m, saveToDisk.m:56,125): a 2-D grid
+   * with x axial and y radial from the axis, run with the WSWA radial
+   * symmetry of kspaceFirstOrderAS.m (inputChecking.m:1400-1411); the radial
+   * PML sits at the outer edge only (getPML.m:73-77); lossless or stokes
+   * absorption (inputChecking.m:165-200). */
+  uint64_t axisymmetric_flag;
 } kw_sim_desc;
 
 typedef struct kw_sim kw_sim;

@@ -480,7 +480,7 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
   }
   Reader R{in};
   // flags this solver does not run (saveToDisk.m:44-64)
-  const char* unsupported[] = {"elastic_flag", "axisymmetric_flag", "nonuniform_grid_flag",
+  const char* unsupported[] = {"elastic_flag", "nonuniform_grid_flag",
                                "sxx_source_flag", "syy_source_flag", "szz_source_flag", "sxy_source_flag",
                                "sxz_source_flag", "syz_source_flag"};
   for (const char* n : unsupported) {
@@ -547,6 +547,7 @@ extern "C" kw_status kw_run_file(const char* input_path, const char* output_path
   }
   // transducer source (saveToDisk.m:290-293, 339-343): the signal, per-point
   // delay_mask, and the active points in u_source_index
+  d.axisymmetric_flag = R.u64("axisymmetric_flag");
   d.transducer_source_flag = R.u64("transducer_source_flag");
   if (d.transducer_source_flag) {
     uint64_t np = 0;

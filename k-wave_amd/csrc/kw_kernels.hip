@@ -650,12 +650,14 @@ __global__ void k_gather(const float* __restrict__ field, const long long* __res
 }
 
 // getPML.m:50-85
-__global__ void k_pml(float* pml, int n, double d, double dt, double c, int size, double alpha, int staggered) {
+// one_sided: the radial axis of axisymmetric grids (getPML.m:73-77)
+__global__ void k_pml(float* pml, int n, double d, double dt, double c, int size, double alpha, int staggered,
+                      int one_sided) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double v = 1.0;
   const double off = staggered ? 0.5 : 0.0;
-  if (j < size) {
+  if (j < size && !one_sided) {
     const double x = j + 1 + off;
     const double r = (x - size - 1) / (0.0 - size);
     v = exp(-alpha * (c / d) * r * r * r * r * dt / 2.0);
@@ -872,8 +874,9 @@ void launch_dvec(hipStream_t s, float2* dpos, float2* dneg, float2* shift, int n
 }
 
 void launch_pml(hipStream_t s, float* pml, int n, double d, double dt, double c, int size, double alpha,
-                int staggered) {
-  hipLaunchKernelGGL(k_pml, dim3(nblk(n, 256)), dim3(256), 0, s, pml, n, d, dt, c, size, alpha, staggered);
+                int staggered, int one_sided) {
+  hipLaunchKernelGGL(k_pml, dim3(nblk(n, 256)), dim3(256), 0, s, pml, n, d, dt, c, size, alpha, staggered,
+                     one_sided);
 }
 
 void launch_spectral_tables(hipStream_t s, const Grid& g, const OpParams& p, float* kappa, float* skappa,

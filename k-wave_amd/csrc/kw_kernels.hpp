@@ -173,6 +173,28 @@ bool launch_x_r2c(hipStream_t s, const Grid& g, const float2* tw, const float* i
 // out[j] = field[lin[j]], j < n
 void launch_gather(hipStream_t s, const float* field, const long long* lin, int n, float* out);
 
+// Axisymmetric radial pass (kw_axisym.hip): tables at the odd frequencies
+// 2m+1 (m < 2 Ny) of the 4 Ny radially expanded grid
+struct AsTables {
+  const float* kappa;    // [m][kx]
+  const float* skappa;   // [m][kx], additive sources only
+  const float2* ydpos;   // i ky exp(+i ky dy/2)
+  const float2* ydneg;   // i ky exp(-i ky dy/2)
+  const float2* yshn;    // exp(-i ky dy/2)
+  const float* ysg_inv;  // 1 / y_vec_sg, Ny values
+  const float2* tw4;     // exp(-2 pi i j / 4Ny)
+  const float2* tw2;     // FFT twiddles of length 2 Ny
+  const float2* dposx;   // x operators of the plain loop (half spectrum)
+  const float2* dnegx;
+};
+void launch_as_tables(hipStream_t s, const Grid& g, double dx, double dy, double dt, double c_ref, const AsTables& T);
+// pencils per workgroup for the radial pass (0 if 2 Ny does not fit lds_max); *lds = bytes
+int as_radial_tile(int Ny, size_t lds_max, size_t* lds);
+// out_o = the radial-spectral operator of a.outs[o] on the x spectra a.in (OP_DPOS_K,
+// OP_DNEG_K on axis 0 / 1, OP_SKAPPA with axis 1 for the uy source), normalised
+void launch_as_radial(hipStream_t s, const Grid& g, const FftAxis& ax, const AsTables& T, const LastArgs& a, int tx,
+                      size_t lds);
+
 // FP64 precompute
 struct OpParams {
   double dx[3];
@@ -182,8 +204,9 @@ struct OpParams {
 };
 void launch_twiddles(hipStream_t s, float2* tw, int n);
 void launch_dvec(hipStream_t s, float2* dpos, float2* dneg, float2* shift, int nfull, int nout, double d);
+// one_sided: the layer at the far end only (the radial axis of axisymmetric grids)
 void launch_pml(hipStream_t s, float* pml, int n, double d, double dt, double c, int size, double alpha,
-                int staggered);
+                int staggered, int one_sided = 0);
 void launch_spectral_tables(hipStream_t s, const Grid& g, const OpParams& p, float* kappa, float* skappa,
                             float* nabla1, float* nabla2);
 void launch_absorb_coeffs(hipStream_t s, long long n, const float* c0, float c0s, const float* alpha,

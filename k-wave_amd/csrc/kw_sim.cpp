@@ -86,9 +86,9 @@ bool make_axis(int n, FftAxis* ax) {
   return m == 1;
 }
 
-enum KernelKind { KK_YPASS, KK_LAST, KK_XVEL, KK_XDENS, KK_XPRES, KK_XINIT, KK_XSRC, KK_PLANE, KK_COUNT };
+enum KernelKind { KK_YPASS, KK_LAST, KK_XVEL, KK_XDENS, KK_XPRES, KK_XINIT, KK_XSRC, KK_PLANE, KK_RADIAL, KK_COUNT };
 const char* kKindNames[KK_COUNT] = {"y_pass", "last_axis", "x_velocity", "x_density",
-                                    "x_pressure", "x_init", "x_source", "yz_plane"};
+                                    "x_pressure", "x_init", "x_source", "yz_plane", "radial"};
 
 struct SrcData {
   int* delay = nullptr;  // transducer: per-point 0-based delay into the signal
@@ -117,6 +117,12 @@ struct kw_sim {
   XLaunch xl{};
   TileLaunch tly{}, tll{};
   int plane_nt = 0;  // 0: per-size default
+  // axisymmetric grids: the radial pass replaces the y transform
+  int axisym = 0;
+  FftAxis axR{};  // length 2 Ny
+  AsTables at{};
+  int as_tx = 0;
+  size_t as_lds = 0;
   // state
   float* u[3] = {};
   float* rho[3] = {};
@@ -320,6 +326,11 @@ void last(kw_sim* s, hipStream_t st, const LastArgs& a) {
 // kx-major spectra: one plane pass; otherwise y pass over the inputs, the
 // last-axis pass, y pass over the outputs.
 void spectral(kw_sim* s, hipStream_t st, const LastArgs& a) {
+  if (s->axisym) {
+    Prof pr(s, st, KK_RADIAL);
+    launch_as_radial(st, s->g, s->axR, s->at, a, s->as_tx, s->as_lds);
+    return;
+  }
   if (s->g.kxm) {
     Prof pr(s, st, KK_PLANE);
     if (!launch_plane(st, s->g, s->twY, s->twL, a, s->T, s->plane_nt))
@@ -332,8 +343,9 @@ void spectral(kw_sim* s, hipStream_t st, const LastArgs& a) {
 }
 
 // Corrected additive source: real(ifft(source_kappa .* fft(scatter))) in X
-// domain, (3D.m:815-822); unit scatters 1.0 (pattern build).
-void source_spectrum(kw_sim* s, hipStream_t st, const Src& src, int unit, float2* out) {
+// domain, (3D.m:815-822); unit scatters 1.0 (pattern build).  comp: the
+// source's component (the radial mirror of an axisymmetric uy source differs).
+void source_spectrum(kw_sim* s, hipStream_t st, const Src& src, int unit, float2* out, int comp = 0) {
   {
     Prof pr(s, st, KK_XSRC);
     launch_x_source(st, s->g, s->axX, s->twX, src, unit, out, s->xl);
@@ -343,7 +355,7 @@ void source_spectrum(kw_sim* s, hipStream_t st, const Src& src, int unit, float2
   a.out[0] = out;
   a.nin = 1;
   a.nout = 1;
-  a.outs[0] = SpecOut{0, OP_SKAPPA, 0};
+  a.outs[0] = SpecOut{0, OP_SKAPPA, comp};
   a.scale = (float)(1.0 / (double)s->g.N);
   spectral(s, st, a);
 }
@@ -490,7 +502,8 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
   Src us_src[3] = {};
   for (int c = 0; c < dim; ++c) {
     us_src[c] = make_src(s, s->usrc[c], t);
-    if (us_src[c].active && us_src[c].mode == 2 && us_src[c].many) source_spectrum(s, st, us_src[c], 0, s->usrc[c].spec);
+    if (us_src[c].active && us_src[c].mode == 2 && us_src[c].many)
+      source_spectrum(s, st, us_src[c], 0, s->usrc[c].spec, c);
   }
   {
     Prof pr(s, st, KK_XVEL);
@@ -537,7 +550,7 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
 kw_status setup_source(kw_sim* s, SrcData& d, uint64_t flag, uint64_t mode, uint64_t many, uint64_t npoints,
                        const uint64_t* index, const float* input, const char* name,
                        const std::vector<long long>* shared_lin, const std::vector<int>* shared_rows,
-                       int* shared_row_dev, long long* shared_lin_dev) {
+                       int* shared_row_dev, long long* shared_lin_dev, int comp = 0) {
   d.flag = flag;
   if (!flag) return KW_OK;
   if (mode > 2) return fail(KW_ERR_INVALID, "%s mode %llu not in 0..2", name, (unsigned long long)mode);
@@ -562,7 +575,7 @@ kw_status setup_source(kw_sim* s, SrcData& d, uint64_t flag, uint64_t mode, uint
       Src u{};
       u.row_ptr = d.row_ptr;
       u.lin = d.lin;
-      source_spectrum(s, s->stream, u, 1, tmp);
+      source_spectrum(s, s->stream, u, 1, tmp, comp);
       d.pattern = s->alloc<float>((size_t)s->g.N);
       launch_x_c2r(s->stream, s->g, s->axX, s->twX, tmp, d.pattern, s->xl);
     }
@@ -707,6 +720,7 @@ uint64_t kw_sim_footprint_bytes(const kw_sim_desc* d) {
   b += src(d->uy_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
   if (dim == 3) b += src(d->uz_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
   if (d->transducer_source_flag) b += d->transducer_source_length * 4 + d->u_source_npoints * 16;
+  if (d->axisymmetric_flag) b += (d->Nx / 2 + 1) * 2 * d->Ny * 8 + d->Ny * 64;  // radial tables
   return b;
 }
 
@@ -739,11 +753,19 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
   if (d->absorbing_flag == 1 && std::fabs(d->alpha_power - 1.0) < 1e-12)
     return fail(KW_ERR_INVALID, "alpha_power = 1 is not supported by the power-law absorption model");
 
+  if (d->axisymmetric_flag) {
+    if (d->Nz > 1) return fail(KW_ERR_INVALID, "axisymmetric inputs are 2-D (x axial, y radial)");
+    if (d->absorbing_flag == 1)
+      return fail(KW_ERR_INVALID, "the axisymmetric code supports stokes absorption only (absorbing_flag 2)");
+    if (d->record_flags & KW_REC_U_NON_STAGGERED)
+      return fail(KW_ERR_UNSUPPORTED, "u_non_staggered is not supported for axisymmetric inputs");
+  }
   // host-side checks of everything the kernels index with, before any HIP call
   const uint64_t psz_chk[3] = {d->pml_x_size, d->pml_y_size, d->Nz > 1 ? d->pml_z_size : 0};
   const uint64_t n_chk[3] = {d->Nx, d->Ny, d->Nz};
   for (int a = 0; a < 3; ++a)
-    if (2 * psz_chk[a] > n_chk[a])
+    // the radial PML of an axisymmetric grid is one-sided (inputChecking.m:1378-1379)
+    if ((d->axisymmetric_flag && a == 1 ? psz_chk[a] : 2 * psz_chk[a]) > n_chk[a])
       return fail(KW_ERR_INVALID, "PML size %llu too large for an axis of %llu points",
                   (unsigned long long)psz_chk[a], (unsigned long long)n_chk[a]);
   if (d->sensor_npoints && !d->sensor_mask_index) return fail(KW_ERR_INVALID, "sensor_mask_index missing");
@@ -836,6 +858,14 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
       const char* pe = std::getenv("KW_PLANE");
       const bool allow = !(pe && pe[0] == '0');
       g.kxm = (allow && need > 0 && need <= (size_t)lds_max && s->xl.nt > 0) ? 1 : 0;
+      if (d->axisymmetric_flag) {
+        s->axisym = 1;
+        s->as_tx = make_axis(2 * g.Ny, &s->axR) ? as_radial_tile(g.Ny, (size_t)lds_max, &s->as_lds) : 0;
+        if (!s->as_tx) {
+          delete s;
+          return fail(KW_ERR_UNSUPPORTED, "axisymmetric radial pass: Ny = %d does not fit the workgroup's LDS", g.Ny);
+        }
+      }
       // kx-major rows stage their spectra through LDS: 256-thread row kernels
       // (measured fastest there; otherwise 256 for Nx >= 256 and 128 below)
       if (g.kxm && !std::getenv("KW_ROW_NT")) s->xl.nt = 256;
@@ -910,10 +940,33 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     for (int a = 0; a < g.dim; ++a) {
       float* pm = s->alloc<float>(nfull[a]);
       float* pms = s->alloc<float>(nfull[a]);
-      launch_pml(st0, pm, nfull[a], dxs[a], d->dt, d->c_ref, (int)psz[a], pal[a], 0);
-      launch_pml(st0, pms, nfull[a], dxs[a], d->dt, d->c_ref, (int)psz[a], pal[a], 1);
+      // axisymmetric: radial layer at the outer edge only (kspaceFirstOrderAS.m:470-471)
+      const int one_sided = s->axisym && a == 1;
+      launch_pml(st0, pm, nfull[a], dxs[a], d->dt, d->c_ref, (int)psz[a], pal[a], 0, one_sided);
+      launch_pml(st0, pms, nfull[a], dxs[a], d->dt, d->c_ref, (int)psz[a], pal[a], 1, one_sided);
       s->m.pml[a] = pm;
       s->m.pml_sg[a] = pms;
+    }
+
+    // axisymmetric radial-pass tables (kspaceFirstOrderAS.m:488-521, 571), FP64 -> FP32
+    if (s->axisym) {
+      const int L = 2 * g.Ny;
+      float2* tw4 = s->alloc<float2>((size_t)2 * L);
+      float2* tw2 = s->alloc<float2>((size_t)L);
+      launch_twiddles(st0, tw4, 2 * L);
+      launch_twiddles(st0, tw2, L);
+      AsTables& at = s->at;
+      at.tw4 = tw4;
+      at.tw2 = tw2;
+      at.kappa = s->alloc<float>((size_t)g.Nxh * L);
+      at.skappa = additive ? s->alloc<float>((size_t)g.Nxh * L) : nullptr;
+      at.ydpos = s->alloc<float2>((size_t)L);
+      at.ydneg = s->alloc<float2>((size_t)L);
+      at.yshn = s->alloc<float2>((size_t)L);
+      at.ysg_inv = s->alloc<float>((size_t)g.Ny);
+      at.dposx = s->T.dpos[0];
+      at.dnegx = s->T.dneg[0];
+      launch_as_tables(st0, g, d->dx, d->dy, d->dt, d->c_ref, at);
     }
 
     // medium
@@ -1040,7 +1093,8 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
       for (int a = 0; a < g.dim; ++a) {
         if (!uf[a]) continue;
         SrcData& sd = s->usrc[a];
-        if (shared_pattern && d->u_source_mode == 2 && !d->u_source_many) {
+        // (axisymmetric: ux and uy patterns differ, their radial mirrors do)
+        if (shared_pattern && d->u_source_mode == 2 && !d->u_source_many && !s->axisym) {
           // the corrected pattern depends only on the (shared) positions
           sd.flag = uf[a];
           sd.mode = 2;
@@ -1055,7 +1109,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
           continue;
         }
         if ((st = setup_source(s, sd, uf[a], d->u_source_mode, d->u_source_many, d->u_source_npoints,
-                               d->u_source_index, uin[a], "u_source", nullptr, nullptr, rd, ld))) {
+                               d->u_source_index, uin[a], "u_source", nullptr, nullptr, rd, ld, a))) {
           free_all(s);
           delete s;
           return st;

@@ -78,6 +78,7 @@ class KwSimDesc(C.Structure):
         ("record_flags", C.c_uint64), ("record_start_index", C.c_uint64),
         ("transducer_source_flag", C.c_uint64), ("transducer_source_length", C.c_uint64),
         ("transducer_source_input", C.POINTER(C.c_float)), ("delay_mask", C.POINTER(C.c_uint64)),
+        ("axisymmetric_flag", C.c_uint64),
     ]
 
 
@@ -254,6 +255,7 @@ def build_desc(f: dict, record=("p",), record_start_index: int = 1, device: int 
         for k in ("ux", "uy", "uz"):
             if _int(f.get(k + "_source_flag")):
                 setattr(d, k + "_source_input", ptr_f(f[k + "_source_input"]))
+    d.axisymmetric_flag = _int(f.get("axisymmetric_flag"))
     d.transducer_source_flag = _int(f.get("transducer_source_flag"))
     if d.transducer_source_flag:
         sig = _f32(f["transducer_source_input"])

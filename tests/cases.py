@@ -215,3 +215,54 @@ def case_axisym(Nx=64, Ny=32, Nt=50, dx=1e-4, source="p0", het=False, absorbing=
     mask[:, 0] = 1
     mask[Nx // 2, :] = 1
     return fe.save_to_disk_dict(kg, medium, src, {"mask": mask}, pml_size=pml, smooth_p0=False, axisymmetric=True)
+
+
+def cw_signal(t, f0, amp, ramp_periods=4):
+    """createCWSignals.m: amp sin(2 pi f0 t) with a cosine up-ramp of 4 periods."""
+    s = amp * np.sin(2 * np.pi * f0 * t)
+    dt = t[1] - t[0]
+    n = int(round(ramp_periods / f0 / dt))
+    ramp = (-np.cos(np.linspace(0, np.pi, n)) + 1) * 0.5
+    s[:n] *= ramp
+    return s
+
+
+def piston_axisym():
+    """testing/unit/kspaceFirstOrderAS_piston_analytical.m: a CW piston of radius
+    5 mm at 1 MHz on the axis of an axisymmetric grid (12 points per wavelength,
+    CFL 0.25, 40 us).  The reference runs it with PMLInside false; here the PML
+    is part of the grid (320 x 100, radices 2 and 5) and the piston sits on the
+    first interior plane.  Returns (input dict, record_start_index, error_fn):
+    error_fn(p) is the L-inf error (% of the peak) of the steady-state on-axis
+    amplitude of the recorded p against P0 |2 sin(k (sqrt(x^2 + a^2) - x) / 2)|."""
+    c0, rho0, f0, radius, mag = 1500.0, 1000.0, 1e6, 5e-3, 1e6
+    ppw, cfl, t_end, periods = 12, 0.25, 40e-6, 3
+    dx = c0 / (ppw * f0)
+    nx_user, ny_user = 256, 80                      # roundEven(32 mm / dx), roundEven(10 mm / dx)
+    pml_x, pml_y = 32, 20
+    Nx, Ny = nx_user + 2 * pml_x, ny_user + pml_y
+    ppp = round(ppw / cfl)
+    dt = 1 / (ppp * f0)
+    Nt = round(t_end / dt)
+    kg = fe.KWaveGrid(Nx, dx, Ny, dx)
+    kg.set_time(Nt, dt)
+    rgp = round(radius / dx)
+    pmask = np.zeros((Nx, Ny))
+    pmask[pml_x, :rgp] = 1
+    sens = np.zeros((Nx, Ny))
+    sens[pml_x + 1:pml_x + nx_user, 0] = 1           # the axis, x = 2 .. end of the user grid
+    sig = cw_signal(np.arange(Nt) * dt, f0, mag)
+    f = fe.save_to_disk_dict(kg, {"sound_speed": c0, "density": rho0}, {"p": sig[None], "p_mask": pmask},
+                             {"mask": sens}, pml_size=[pml_x, pml_y], axisymmetric=True)
+    start = Nt - periods * ppp + 1
+
+    def error_pct(p):
+        n = p.shape[0]
+        amp = 2 / n * np.abs(np.exp(-2j * np.pi * f0 * dt * np.arange(n)) @ p)   # 3 whole periods, rectangular
+        x = np.arange(1, nx_user) * dx
+        k = 2 * np.pi * f0 / c0
+        a = (rgp - 0.5) * dx
+        ref = mag * np.abs(2 * np.sin((k * np.sqrt(x ** 2 + a ** 2) - k * x) / 2))
+        return 100 * np.max(np.abs(ref - amp)) / np.max(ref)
+
+    return f, start, error_pct

@@ -218,6 +218,7 @@ def test_binary_missing_input_file(kw, tmp_path):
     (lambda f: f.update(sensor_mask_type=np.array([[[2]]], dtype=np.uint64)), "sensor_mask_type"),
     (lambda f: f.pop("c0"), "c0"),
     (lambda f: f.update(p_source_input=np.zeros((1, 3, 1), dtype=np.float32)), "p_source_input"),
+    (lambda f: f.update(axisymmetric_flag=np.array([[[1]]], dtype=np.uint64)), "stokes absorption only"),
 ])
 def test_binary_rejects_inputs_it_cannot_run(kw, tmp_path, mutate, msg):
     h5 = _binary(kw)
@@ -284,3 +285,32 @@ def test_labelled_source_front_end_expansion(kw):
     with pytest.raises(ValueError, match="number of time series"):
         fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0},
                              {"p": sig[:1].repeat(3, 0), "p_mask": m}, {"mask": m > 0}, pml_size=4)
+
+
+def test_axisymmetric_front_end(kw):
+    """kspaceFirstOrderASC's file: axisymmetric_flag 1, stokes absorption with
+    y = 2 (inputChecking.m:165-200), a one-sided radial PML that may fill up to
+    Ny (inputChecking.m:1378-1379), p0 smoothed on the WSWA-mirrored grid
+    (inputChecking.m:1505-1534): smoothing a field that is already the trim of
+    a mirrored band-limited one leaves its shape, a plain smooth does not."""
+    fe = kw.frontend
+    kg = fe.KWaveGrid(32, 1e-4, 12, 1e-4)
+    kg.set_time(5, 2e-8)
+    p0 = np.zeros((32, 12))
+    p0[10:14, 0:3] = 1.0
+    f = fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0, "alpha_coeff": 0.5, "alpha_power": 1.5},
+                             {"p0": p0}, {"mask": np.ones((32, 12))}, pml_size=[4, 12], axisymmetric=True)
+    assert int(f["axisymmetric_flag"].ravel()[0]) == 1 and int(f["absorbing_flag"].ravel()[0]) == 2
+    assert float(f["alpha_power"].ravel()[0]) == 2.0
+    e = np.zeros((32, 48))
+    e[:, :12] = p0
+    e[:, 13:24] = -p0[:, :0:-1]
+    e[:, 24:36] = -p0
+    e[:, 37:] = p0[:, :0:-1]
+    want = kw.geometry.smooth(e, True)[:, :12]
+    assert np.allclose(f["p0_source_input"][0].T, want, atol=1e-6 * np.abs(want).max())
+    plain = kw.geometry.smooth(p0, True)
+    assert not np.allclose(want, plain, atol=1e-3)
+    with pytest.raises(ValueError):
+        fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0}, {"mask": np.ones((32, 12))},
+                             pml_size=[4, 13], axisymmetric=True)

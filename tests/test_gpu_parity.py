@@ -210,3 +210,27 @@ def test_labelled_source_mask(kw, oracle, kind, mode):
     """Labelled p_mask / u_mask (inputChecking.m:847-884, 964-1003) in a heterogeneous medium."""
     f = cases.case_labelled(kind, mode=mode, het=True)
     compare(kw, oracle, f, ("p", "p_max", "u_final"), f"labelled_{kind}_{mode}")
+
+
+AXISYM_CASES = [
+    ("p0", dict(source="p0")),
+    ("p0_het_stokes_nonlinear", dict(source="p0", het=True, absorbing=True, nonlinear=True)),
+    ("p_additive", dict(source="p")),
+    ("p_dirichlet_het", dict(source="p", mode="dirichlet", het=True)),
+    ("p_no_correction", dict(source="p", mode="additive-no-correction")),
+    ("u_additive_het_stokes", dict(source="u", het=True, absorbing=True)),
+    ("u_dirichlet", dict(source="u", mode="dirichlet")),
+]
+
+
+@pytest.mark.parametrize("name,kwargs", AXISYM_CASES, ids=[c[0] for c in AXISYM_CASES])
+def test_axisymmetric(kw, oracle, name, kwargs):
+    """kspaceFirstOrderAS (WSWA radial symmetry) through the C ABI against the oracle."""
+    f = cases.case_axisym(Nx=64, Ny=48, Nt=60, **kwargs)
+    compare(kw, oracle, f, ("p", "p_max", "p_rms", "u", "u_final", "p_final"), f"axisym_{name}")
+
+
+def test_axisymmetric_odd_lengths(kw, oracle):
+    """Generic x rows (Nx = 70) and a radial length 2 Ny = 90 with radix 3 and 5 stages."""
+    f = cases.case_axisym(Nx=70, Ny=45, Nt=40, source="p", het=True, pml=6)
+    compare(kw, oracle, f, ("p", "u_final"), "axisym_odd")
