@@ -6,8 +6,9 @@
 * run_binary(in, out,...) -- the kspaceFirstOrder-HIP executable through a process
                            boundary, as kspaceFirstOrder3DC.m:544 runs it
 * kspace_first_order(kgrid, medium, source, sensor, ...) -- the
-  kspaceFirstOrder2DG/3DG flow: SaveToDisk input, run the binary, read back
-  sensor_data shaped as MATLAB returns it (sensor_data.p is [Ns, Nt]).
+  kspaceFirstOrder2DG/3DG (and, axisymmetric=True, kspaceFirstOrderASC) flow:
+  SaveToDisk input, run the binary, read back sensor_data shaped as MATLAB
+  returns it (sensor_data.p is [Ns, Nt]).
 """
 
 from __future__ import annotations
@@ -216,10 +217,11 @@ def sensor_data_from_output(o: dict, record, dim: int) -> dict:
 
 def kspace_first_order(kgrid, medium, source, sensor, record=("p",), record_start_index=None, pml_size=None,
                        pml_alpha=2.0, smooth_p0=True, data_path=None, device=None, delete_data=True,
-                       binary=BINARY_PATH):
-    """kspaceFirstOrder2DG/3DG: write the input file, run the binary, read the output."""
+                       binary=BINARY_PATH, axisymmetric=False):
+    """kspaceFirstOrder2DG/3DG (kspaceFirstOrderASC with axisymmetric=True): write
+    the input file, run the binary, read the output."""
     f = frontend.save_to_disk_dict(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha,
-                                   smooth_p0=smooth_p0)
+                                   smooth_p0=smooth_p0, axisymmetric=axisymmetric)
     d = data_path or tempfile.mkdtemp(prefix="kwave_")
     inp, out = os.path.join(d, "kwave_input.h5"), os.path.join(d, "kwave_output.h5")
     write_input(inp, f)

@@ -227,3 +227,38 @@ def test_transducer_through_the_file(kw, tmp_path):
     direct = kw.run(f, record=("p", "u_final"))
     for k in direct:
         np.testing.assert_array_equal(o[k].reshape(direct[k].shape), direct[k])
+
+
+def test_axisymmetric_through_the_binary(kw, oracle, tmp_path):
+    """kspaceFirstOrderASC's file (axisymmetric_flag = 1) through the executable:
+    bit-equal to the in-process run, within 1e-4 of the oracle; and the
+    kspace_first_order(axisymmetric=True) flow returns MATLAB-shaped data."""
+    from kwave_amd import h5format
+
+    print("[gpu binary] axisymmetric u additive stokes via kspaceFirstOrder-HIP", flush=True)
+    f = cases.case_axisym(Nx=64, Ny=48, Nt=50, source="u", het=True, absorbing=True)
+    inp, out = str(tmp_path / "in.h5"), str(tmp_path / "out.h5")
+    h5format.write_input(inp, f)
+    rec = ("p", "p_max", "u_final", "p_final")
+    r = h5format.run_binary(inp, out, rec)
+    assert r.returncode == 0, r.stderr
+    o = h5format.read_file(out)
+    direct = kw.run(f, record=rec)
+    for k in direct:
+        np.testing.assert_array_equal(o[k].reshape(direct[k].shape), direct[k])
+    ref = oracle.run(f, record=rec)
+    for k in ref:
+        assert cases.rel_l2(o[k].reshape(ref[k].shape), ref[k]) < 1e-4, k
+    fe = cases.fe
+    kg = fe.KWaveGrid(64, 1e-4, 32, 1e-4)
+    kg.set_time(40, 2e-8)
+    p0 = np.zeros((64, 32))
+    p0[30:34, :3] = 1.0
+    sensor = np.zeros((64, 32))
+    sensor[:, 0] = 1
+    sd = h5format.kspace_first_order(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0}, {"mask": sensor},
+                                     record=("p",), pml_size=8, axisymmetric=True)
+    assert sd["p"].shape == (64, 40)
+    fa = fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0}, {"mask": sensor},
+                              pml_size=8, axisymmetric=True)
+    assert cases.rel_l2(sd["p"].T, oracle.run(fa, record=("p",))["p"]) < 1e-4
