@@ -198,7 +198,8 @@ int kw_sim_profile_get(kw_sim* sim, const char** names, double* total_ms, uint64
  * the row, pencil and plane kernels over `trial_steps` steps of this
  * simulation, keeps the fastest (cached per grid shape and physics branch for
  * the rest of the process) and resets the simulation to step 0.  Call before
- * kw_sim_run.  The chosen sizes are returned (0: kernel kind not used).
+ * kw_sim_run.  The chosen sizes are returned (0: kernel kind not used;
+ * pencil_nt is the y pass's, or the last axis's when there is no y pass).
  * No counterpart in the reference (its binaries have fixed launch shapes). */
 kw_status kw_sim_autotune(kw_sim* sim, int trial_steps, int* row_nt, int* pencil_nt, int* plane_nt);
 

@@ -753,14 +753,17 @@ bool fast_length(int n) {
   ((nt) == 64 ? fn<64>(__VA_ARGS__)                             \
               : (nt) == 128 ? fn<128>(__VA_ARGS__) : (nt) == 256 ? fn<256>(__VA_ARGS__) : false)
 
+// pencils also have 512-thread instantiations
+#define KW_BY_NT_PENCIL(nt, fn, ...) ((nt) == 512 ? fn<512>(__VA_ARGS__) : KW_BY_NT(nt, fn, __VA_ARGS__))
+
 static bool ypass_fast(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign,
                        int nt) {
-  return KW_BY_NT(nt, launch_ypass_fast_t, s, g, n, tw, f, nf, sign);
+  return KW_BY_NT_PENCIL(nt, launch_ypass_fast_t, s, g, n, tw, f, nf, sign);
 }
 
 static bool last_fast(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T,
                       int nt) {
-  return KW_BY_NT(nt, launch_last_fast_t, s, g, n, tw, a, T);
+  return KW_BY_NT_PENCIL(nt, launch_last_fast_t, s, g, n, tw, a, T);
 }
 
 static size_t x_smem(const Grid& g, int P, int nbuf, bool slot) {

@@ -112,7 +112,8 @@ struct TileLaunch {
 };
 
 // Axis lengths with compiled length-specialised kernels (kw_pencil_nt*.hip,
-// kw_xrow_nt*.hip, instantiated for 64-, 128- and 256-thread workgroups).
+// kw_xrow_nt*.hip, instantiated for 64-, 128- and 256-thread workgroups; the
+// pencils also for 512).
 #define KW_FAST_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(160) X(192) X(256) X(384) X(512) X(768)
 bool fast_length(int n);
 
