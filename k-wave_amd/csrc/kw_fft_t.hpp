@@ -17,8 +17,11 @@
 
 namespace kwt {
 
+// 16 is split 4 x 4 when it would be the last stage of a longer transform
+// (256 = 16 x 4 x 4): the last stage's radix sets the registers each thread
+// holds between a C2R and its R2C, and a radix-16 last stage costs 16 of them.
 constexpr int next_radix(int m) {
-  return (m % 16 == 0) ? 16 : (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2
+  return (m == 16) ? 4 : (m % 16 == 0) ? 16 : (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2
        : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : 0;
 }
 constexpr int num_stages(int n) {
