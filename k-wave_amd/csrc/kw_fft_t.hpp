@@ -251,6 +251,17 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
         }
       }
       bfly<R>(v[it], sign);
+    }
+  }
+  // stores after every butterfly: a global store between two iterations'
+  // twiddle loads would make each wait for the other's round trip
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (TMAX % NT == 0 || w < TMAX) {
+      const int t = TFAST ? w % CMAX : w / NB;
+      const int i = TFAST ? w / CMAX : w % NB;
+      const int k = i % NS;
       const int j = (i / NS) * NS * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) st(t, j + r * NS, v[it][r], it * R + r);
