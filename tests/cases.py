@@ -266,3 +266,46 @@ def piston_axisym():
         return 100 * np.max(np.abs(ref - amp)) / np.max(ref)
 
     return f, start, error_pct
+
+
+def fourier_shift(data, shift):
+    """fourierShift.m along the last axis: real(ifft(ifftshift(exp(i k shift)) .* fft(data)))."""
+    n = data.shape[-1]
+    k = 2 * np.pi * (np.arange(-(n // 2), n - n // 2) / n)
+    k[n // 2] = 0.0
+    return np.real(np.fft.ifft(np.fft.ifftshift(np.exp(1j * k * shift)) * np.fft.fft(data, axis=-1), axis=-1))
+
+
+def plane_wave_intensity():
+    """testing/unit/kspaceFirstOrder_check_plane_wave_intensity.m, 2-D x case: a
+    10 MHz plane wave of 10 W/cm^2 from a p source line; the time-averaged
+    intensity p .* fourierShift(ux_non_staggered, 1/2) (saveIntensity.m) over the
+    last two periods at a point across the grid must be 10 W/cm^2 within 1 %.
+    Returns (input dict, record_start_index, error_fn(p, ux_non_staggered) -> %)."""
+    Nx, Nj, dx = 64, 32, 0.05e-3
+    c0, rho0, f0, I0 = 1500.0, 1000.0, 10e6, 10.0
+    strength = np.sqrt(2 * I0 * 1e4 * rho0 * c0)
+    ppp = 30
+    dt = 1 / f0 / (2 * ppp)
+    Nt = int(np.floor(3e-6 / dt + 1e-9)) + 1
+    t = np.arange(Nt) * dt
+    sig = strength * np.sin(2 * np.pi * f0 * t)
+    ramp = (-np.cos(np.arange(200) * np.pi / 200) + 1) / 2
+    sig[:200] *= ramp
+    kg = fe.KWaveGrid(Nx, dx, Nj, dx)
+    kg.set_time(Nt, dt)
+    pm = np.zeros((Nx, Nj))
+    pm[20, :] = 1                                    # source_position = PML_size + 1
+    sm = np.zeros((Nx, Nj))
+    sm[Nx - 22, Nj // 2 - 1] = 1                     # sensor_position = Nx - PML_size - 1, Nj/2
+    f = fe.save_to_disk_dict(kg, {"sound_speed": c0, "density": rho0}, {"p": sig[None], "p_mask": pm},
+                             {"mask": sm}, pml_size=[20, 2], pml_alpha=[2.0, 0.0])
+    start = Nt - 2 * ppp + 1
+
+    def error_pct(p, ux_ns):
+        p = np.asarray(p, dtype=np.float64).reshape(-1)
+        u = np.asarray(ux_ns, dtype=np.float64).reshape(-1)
+        I_avg = np.mean(p * fourier_shift(u, 0.5)) / 1e4      # W/cm^2
+        return 100 * abs(I_avg - I0) / I0
+
+    return f, start, error_pct

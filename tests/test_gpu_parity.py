@@ -234,3 +234,14 @@ def test_axisymmetric_odd_lengths(kw, oracle):
     """Generic x rows (Nx = 70) and a radial length 2 Ny = 90 with radix 3 and 5 stages."""
     f = cases.case_axisym(Nx=70, Ny=45, Nt=40, source="p", het=True, pml=6)
     compare(kw, oracle, f, ("p", "u_final"), "axisym_odd")
+
+
+def test_plane_wave_intensity(kw, oracle):
+    """testing/unit/kspaceFirstOrder_check_plane_wave_intensity.m through the C ABI:
+    the intensity formed from p and ux_non_staggered is the source's 10 W/cm^2
+    within 1 %, and both series match the oracle."""
+    f, start, error_pct = cases.plane_wave_intensity()
+    gpu, _ = compare(kw, oracle, f, ("p", "u_non_staggered"), "plane_wave_intensity", start=start)
+    err = error_pct(gpu["p"], gpu["ux_non_staggered"])
+    print(f"    intensity error {err:.3f} %", flush=True)
+    assert err < 1.0

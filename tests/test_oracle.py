@@ -536,3 +536,14 @@ def test_axisymmetric_piston_on_axis_amplitude(oracle):
     f, start, error_pct = cases.piston_axisym()
     p = oracle.run(f, record=("p",), record_start_index=start)["p"]
     assert error_pct(p) < 1.0
+
+
+def test_plane_wave_intensity_from_u_non_staggered(oracle):
+    """testing/unit/kspaceFirstOrder_check_plane_wave_intensity.m (2-D x case):
+    the intensity the MATLAB wrapper forms from p and ux_non_staggered
+    (kspaceFirstOrder_saveIntensity.m) equals the source intensity within 1 %;
+    pins u_non_staggered's amplitude and phase against p, and the p source scaling."""
+    import cases
+    f, start, error_pct = cases.plane_wave_intensity()
+    out = oracle.run(f, record=("p", "u_non_staggered"), record_start_index=start)
+    assert error_pct(out["p"], out["ux_non_staggered"]) < 1.0
