@@ -79,6 +79,12 @@ def k_vec_unshifted(n: int, d: float) -> np.ndarray:
     return np.fft.ifftshift(make_dim(n, d))
 
 
+def x_shift_neg(n: int, d: float) -> np.ndarray:
+    """record.x_shift_neg = ifftshift(exp(-1i kx_vec dx / 2)), the half-cell shift
+    of the u_non_staggered outputs.  kspaceFirstOrder_createStorageVariables.m:87-99."""
+    return np.exp(-1j * k_vec_unshifted(n, d) * d / 2)
+
+
 def sinc(x):
     """sin(x)/x with sinc(0)=1.  k-Wave/private/sinc.m."""
     x = np.asarray(x, dtype=np.float64)
@@ -470,7 +476,7 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
         shift_neg = []
         for a in range(dim):
             n = shape[2 - a]
-            v = np.exp(-1j * k_vec_unshifted(n, d[a]) * d[a] / 2)
+            v = x_shift_neg(n, d[a])
             sh = [1, 1, 1]
             sh[2 - a] = n
             shift_neg.append(v.reshape(sh))

@@ -309,3 +309,30 @@ def plane_wave_intensity():
         return 100 * abs(I_avg - I0) / I0
 
     return f, start, error_pct
+
+
+def nonstaggered_velocity_order():
+    """testing/unit/kspaceFirstOrder2D_check_nonstaggered_particle_velocity.m: a
+    smoothed disc p0 (radius 5, magnitude 5) at the centre of a 128^2 absorbing
+    grid (0.75 dB/(MHz^1.5 cm)), CFL 0.1, 6 us; sensors 40 and 41 points along
+    +x.  Returns the input dict; the check orders the peaks of ux_non_staggered
+    and ux (half a cell further out) at the two points."""
+    Nx = Ny = 128
+    kg = fe.KWaveGrid(Nx, 0.1e-3, Ny, 0.1e-3)
+    medium = {"sound_speed": 1500.0, "density": 1000.0, "alpha_coeff": 0.75, "alpha_power": 1.5}
+    kg.make_time(1500.0, cfl=0.1, t_end=6e-6)
+    p0 = 5.0 * geo.make_disc(Nx, Ny, Nx // 2, Ny // 2, 5)
+    m = np.zeros((Nx, Ny))
+    m[Nx // 2 + 40 - 1, Ny // 2 - 1] = 1
+    m[Nx // 2 + 41 - 1, Ny // 2 - 1] = 1
+    return fe.save_to_disk_dict(kg, medium, {"p0": p0}, {"mask": m})
+
+
+def velocity_order_holds(ux_ns, ux):
+    """max1 >= max2 >= max3 >= max4 and ind1 <= ind2 <= ind3 <= ind4 for
+    (ux_ns at point 1, ux at point 1, ux_ns at point 2, ux at point 2); series
+    are (Nt, 2) as recorded."""
+    s = [ux_ns[:, 0], ux[:, 0], ux_ns[:, 1], ux[:, 1]]
+    mx = [float(np.max(v)) for v in s]
+    ix = [int(np.argmax(v)) for v in s]
+    return all(mx[i] >= mx[i + 1] for i in range(3)) and all(ix[i] <= ix[i + 1] for i in range(3)), mx, ix

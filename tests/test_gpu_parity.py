@@ -245,3 +245,11 @@ def test_plane_wave_intensity(kw, oracle):
     err = error_pct(gpu["p"], gpu["ux_non_staggered"])
     print(f"    intensity error {err:.3f} %", flush=True)
     assert err < 1.0
+
+
+def test_nonstaggered_velocity_peaks_are_ordered(kw, oracle):
+    """kspaceFirstOrder2D_check_nonstaggered_particle_velocity.m through the C ABI."""
+    f = cases.nonstaggered_velocity_order()
+    gpu, _ = compare(kw, oracle, f, ("u", "u_non_staggered"), "nonstaggered_velocity_order")
+    ok, mx, ix = cases.velocity_order_holds(gpu["ux_non_staggered"], gpu["ux"])
+    assert ok, (mx, ix)

@@ -547,3 +547,14 @@ def test_plane_wave_intensity_from_u_non_staggered(oracle):
     f, start, error_pct = cases.plane_wave_intensity()
     out = oracle.run(f, record=("p", "u_non_staggered"), record_start_index=start)
     assert error_pct(out["p"], out["ux_non_staggered"]) < 1.0
+
+
+def test_nonstaggered_velocity_peaks_are_ordered(oracle):
+    """kspaceFirstOrder2D_check_nonstaggered_particle_velocity.m: going outward from
+    a disc source, the peak of ux_non_staggered at a point, of ux (half a cell
+    further), of ux_non_staggered one point on and of ux after it decrease and
+    arrive in that order; a shift of the wrong sign or size breaks the order."""
+    import cases
+    out = oracle.run(cases.nonstaggered_velocity_order(), record=("u", "u_non_staggered"))
+    ok, mx, ix = cases.velocity_order_holds(out["ux_non_staggered"], out["ux"])
+    assert ok, (mx, ix)

@@ -62,3 +62,17 @@ def test_source_scaling_factor_breaks_amplitude(kw, oracle, monkeypatch):
 
     monkeypatch.setattr(fe, "save_to_disk_dict", halved)
     _fails(pins.test_additive_source_scaling_against_finer_dt, kw, oracle)
+
+
+@pytest.mark.parametrize("mistake", ["sign", "none"])
+def test_wrong_non_staggered_shift_breaks_intensity_and_order(oracle, monkeypatch, mistake):
+    """A shift of the wrong sign in u_non_staggered fails both its pins; no shift
+    at all fails the intensity pin (the peak order then only ties, which the
+    reference's order check accepts too)."""
+    if mistake == "sign":
+        monkeypatch.setattr(oracle, "x_shift_neg", lambda n, d: np.exp(1j * oracle.k_vec_unshifted(n, d) * d / 2))
+    else:
+        monkeypatch.setattr(oracle, "x_shift_neg", lambda n, d: np.ones(n, dtype=complex))
+    _fails(pins.test_plane_wave_intensity_from_u_non_staggered, oracle)
+    if mistake == "sign":
+        _fails(pins.test_nonstaggered_velocity_peaks_are_ordered, oracle)
