@@ -258,6 +258,15 @@ def test_plane_wave_intensity(kw, oracle):
 def test_nonstaggered_velocity_peaks_are_ordered(kw, oracle):
     """kspaceFirstOrder2D_check_nonstaggered_particle_velocity.m through the C ABI."""
     f = cases.nonstaggered_velocity_order()
-    gpu, _ = compare(kw, oracle, f, ("u", "u_non_staggered"), "nonstaggered_velocity_order")
+    print("[gpu parity] nonstaggered_velocity_order", flush=True)
+    rec = ("u", "u_non_staggered")
+    gpu, ref = kw.run(f, record=rec), oracle.run(f, record=rec)
+    for k in ("ux", "uy", "ux_non_staggered"):
+        e = cases.rel_l2(gpu[k], ref[k])
+        print(f"    {k}: rel L2 {e:.2e}", flush=True)
+        assert e <= TOL_L2, (k, e)
+    # the sensors sit on the disc's symmetry axis, where uy_non_staggered is
+    # zero up to rounding in both solvers: bounded by ux's peak instead
+    assert np.abs(gpu["uy_non_staggered"]).max() <= 1e-4 * np.abs(gpu["ux_non_staggered"]).max()
     ok, mx, ix = cases.velocity_order_holds(gpu["ux_non_staggered"], gpu["ux"])
     assert ok, (mx, ix)
