@@ -242,11 +242,33 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
         f["p0_source_input"] = _c_order(p0).astype(np.float32)
 
     mode_code = {"dirichlet": 0, "additive-no-correction": 1, "additive": 2}
+
+    def w_correction(field):
+        """source.<field>_frequency_ref (inputChecking.m:818-833, 921-936): the
+        signal is multiplied by cos(2 pi f_ref dt / 2) (scaleSourceTerms.m:41-42,
+        270-278) and the k-space source correction is switched off."""
+        f_ref = source.get(field + "_frequency_ref")
+        if f_ref is None:
+            return None
+        f_ref = float(f_ref)
+        # kgrid.k_max: the smallest of max|k| over the axes (kWaveGrid.m:410-433)
+        k_max = min(2 * math.pi / dd * ((n // 2) / n if n % 2 == 0 else ((n - 1) / 2) / n)
+                    for n, dd in zip(shape, d[:dim]))
+        c_min = float(np.min(c0))
+        if f_ref < 0 or f_ref > k_max * c_min / (2 * np.pi):
+            raise ValueError(f"source.{field}_frequency_ref is higher than the maximum frequency supported by the "
+                             "spatial grid.")
+        return math.cos(2 * math.pi * f_ref * dt / 2)
+
     # pressure source (scaleSourceTerms.m:50-127)
     if source.get("p") is not None:
         sig = np.atleast_2d(np.asarray(source["p"], dtype=np.float64)).copy()
         idx = _find(source["p_mask"])
         mode = source.get("p_mode", "additive")
+        wc = w_correction("p")
+        if wc is not None:
+            sig *= wc
+            mode = "additive-no-correction"
         for r in range(sig.shape[0]):
             c_here = c_at(idx[r]) if hetero_c else c0_s
             if mode == "dirichlet":
@@ -270,6 +292,9 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
     if u_keys:
         idx = _find(source["u_mask"])
         mode = source.get("u_mode", "additive")
+        wcu = w_correction("u")
+        if wcu is not None:
+            mode = "additive-no-correction"
         labels = None
         for k in u_keys:
             labels = _source_labels(source["u_mask"], np.atleast_2d(np.asarray(source[k])).shape[0], k, "u_mask")
@@ -278,6 +303,8 @@ def save_to_disk_dict(kgrid: KWaveGrid, medium: dict, source: dict, sensor: dict
             if source.get(k) is None:
                 continue
             sig = np.atleast_2d(np.asarray(source[k], dtype=np.float64)).copy()
+            if wcu is not None:
+                sig *= wcu
             if mode != "dirichlet":
                 for r in range(sig.shape[0]):
                     c_here = c_at(idx[r]) if hetero_c else c0_s

@@ -336,3 +336,32 @@ def velocity_order_holds(ux_ns, ux):
     mx = [float(np.max(v)) for v in s]
     ix = [int(np.argmax(v)) for v in s]
     return all(mx[i] >= mx[i + 1] for i in range(3)) and all(ix[i] <= ix[i + 1] for i in range(3)), mx, ix
+
+
+def w_correction_plane_wave(kind="p", variant="k", fine=False, Nx=64, Ny=8, f0=1e6, cfl=0.5, sc=10):
+    """kspaceFirstOrder*_check_source_scaling_*.m in miniature: a plane wave from a
+    source line across a periodic y axis (no y PML), recorded 40 points away.
+    variant "k": additive (k-space corrected); "w": additive with
+    p_frequency_ref / u_frequency_ref = f0 (cos(2 pi f0 dt/2) scaling, no
+    k-space correction); "none": additive-no-correction.  fine=True: dt / sc with no
+    correction, the reference the coarse runs are judged against."""
+    dx = 1e-4
+    kg = fe.KWaveGrid(Nx, dx, Ny, dx)
+    dt = cfl * dx / 1500.0 / (sc if fine else 1)
+    Nt = int(round(12e-6 / dt))
+    kg.set_time(Nt, dt)
+    t = np.arange(Nt) * dt
+    burst = np.sin(2 * np.pi * f0 * t) * np.where(t < 3 / f0, np.sin(np.pi * t * f0 / 3) ** 2, 0.0)
+    m = np.zeros((Nx, Ny))
+    m[12, :] = 1
+    sens = np.zeros((Nx, Ny))
+    sens[52, 0] = 1
+    mode = "additive" if (variant == "k" and not fine) else "additive-no-correction"
+    if kind == "p":
+        src = {"p": burst[None] * 1e5, "p_mask": m, "p_mode": mode}
+    else:
+        src = {"ux": burst[None] * 0.1, "u_mask": m, "u_mode": mode}
+    if variant == "w" and not fine:
+        src[("p" if kind == "p" else "u") + "_frequency_ref"] = f0
+    return fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0}, src, {"mask": sens},
+                                pml_size=[10, 0])

@@ -314,3 +314,20 @@ def test_axisymmetric_front_end(kw):
     with pytest.raises(ValueError):
         fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0}, {"mask": np.ones((32, 12))},
                              pml_size=[4, 13], axisymmetric=True)
+
+
+def test_frequency_ref_source_front_end(kw):
+    """source.p_frequency_ref / u_frequency_ref (inputChecking.m:818-833, 921-936,
+    scaleSourceTerms.m:41-42, 270-278): the signal is scaled by cos(2 pi f dt/2)
+    and the mode becomes additive-no-correction; a reference frequency above
+    k_max c_min / 2 pi is rejected."""
+    import cases
+    fk = cases.w_correction_plane_wave("p", variant="k")
+    fw = cases.w_correction_plane_wave("p", variant="w")
+    dt = float(fk["dt"].ravel()[0])
+    assert int(fw["p_source_mode"].ravel()[0]) == 1 and int(fk["p_source_mode"].ravel()[0]) == 2
+    assert np.allclose(fw["p_source_input"], fk["p_source_input"] * np.cos(np.pi * 1e6 * dt), rtol=1e-6)
+    fu = cases.w_correction_plane_wave("ux", variant="w")
+    assert int(fu["u_source_mode"].ravel()[0]) == 1
+    with pytest.raises(ValueError, match="maximum frequency"):
+        cases.w_correction_plane_wave("p", variant="w", f0=8e6)

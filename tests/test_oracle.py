@@ -558,3 +558,26 @@ def test_nonstaggered_velocity_peaks_are_ordered(oracle):
     out = oracle.run(cases.nonstaggered_velocity_order(), record=("u", "u_non_staggered"))
     ok, mx, ix = cases.velocity_order_holds(out["ux_non_staggered"], out["ux"])
     assert ok, (mx, ix)
+
+
+@pytest.mark.parametrize("kind", ["p", "ux"])
+def test_source_corrections_rank_against_fine_dt(oracle, kind):
+    """kspaceFirstOrder2D_check_source_scaling_p/_ux.m (and the AS variants) in
+    miniature: at CFL 0.5 the k-space corrected additive source matches a 10x
+    finer-dt reference best, the frequency-referenced (w) correction next, no
+    correction worst (err_k < err_w < err_none); the w correction is the front
+    end's cos(2 pi f_ref dt/2) scaling with the k-space correction switched off."""
+    import cases
+    fine = oracle.run(cases.w_correction_plane_wave(kind, fine=True), record=("p",))["p"][:, 0]
+    # coarse sample n against the fine series at the same time: fine sample
+    # 10n+9 for a velocity source; a pressure source enters the density update,
+    # which the staggered time grid places half a step off the velocity one, so
+    # its comparison point is midway between fine samples 10n+4 and 10n+5
+    ref = 0.5 * (fine[4::10] + fine[5::10]) if kind == "p" else fine[9::10]
+    err = {}
+    for v in ("k", "w", "none"):
+        out = oracle.run(cases.w_correction_plane_wave(kind, variant=v), record=("p",))["p"][:, 0]
+        n = min(out.size, ref.size)
+        err[v] = np.max(np.abs(out[:n] - ref[:n])) / np.max(np.abs(fine))
+    assert err["k"] < err["w"] < err["none"], err
+    assert err["k"] < 1e-4, err
