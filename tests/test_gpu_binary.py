@@ -76,7 +76,7 @@ def test_matlab_style_front_end(kw, oracle):
     assert cases.rel_l2(sd["p_final"], ref["p_final"][0].T) < 1e-4
 
 
-@pytest.mark.parametrize("dim", [3, 2])
+@pytest.mark.parametrize("dim", [3, 2, "as"])
 def test_cuboid_corners_equal_binary_mask(kw, tmp_path, dim):
     """Cuboid-corner sensors through the file path: /p/<k> etc. hold, per cuboid and in
     [cx, cy, cz, Nt] order, exactly the binary-mask run's values at those points
@@ -88,10 +88,17 @@ def test_cuboid_corners_equal_binary_mask(kw, tmp_path, dim):
         f = cases.case_3d(N=(32, 28, 24), Nt=30)
         shape = (32, 28, 24)
         corners = np.array([[5, 4, 3, 12, 9, 7], [10, 8, 6, 20, 14, 10]]).T      # overlapping
-    else:
+    elif dim == 2:
         f = cases.case_2d_absorbing_psource(N=(60, 48), Nt=40)
         shape = (60, 48)
         corners = np.array([[10, 12, 30, 20], [25, 5, 40, 15]]).T
+    else:
+        # kspaceFirstOrderAS_compare_binary_and_cuboid_sensor_mask.m: the same on an
+        # axisymmetric file, one cuboid touching the axis
+        f = cases.case_axisym(Nx=64, Ny=48, Nt=40, source="p", het=True)
+        shape = (64, 48)
+        corners = np.array([[10, 1, 30, 12], [25, 5, 40, 15]]).T
+        dim = 2
     rec = ("p", "p_max", "p_rms", "u", "p_final")
     fc = dict(f)
     fc.pop("sensor_mask_index")
@@ -174,6 +181,48 @@ def test_time_reversal_through_the_contract(kw, oracle, tmp_path):
     cc = np.corrcoef(got[inside], p0s[inside])[0, 1]
     print(f"    correlation with p0 inside the ring {cc:.2f}", flush=True)
     assert cc > 0.5
+
+
+def test_axisymmetric_time_reversal_through_the_contract(kw, oracle, tmp_path):
+    """kspaceFirstOrderAS_time_reversal.m through the file contract: an on-axis p0
+    recorded on a closed axisymmetric surface (a cylinder at r = 34 with end caps),
+    then the flipped recordings as a dirichlet p source and p_final through the
+    executable on an axisymmetric file; p_final matches the oracle and
+    reconstructs the (smoothed) p0."""
+    from kwave_amd import h5format
+
+    print("[gpu binary] axisymmetric time reversal", flush=True)
+    fe = kw.frontend
+    Nx, Ny, d = 96, 48, 1e-4
+    kg = fe.KWaveGrid(Nx, d, Ny, d)
+    kg.set_time(300, 0.3 * d / 1500.0)
+    x = (np.arange(Nx) - 40)[:, None]
+    r = np.arange(Ny)[None, :]
+    p0 = 2.0 * ((x ** 2 + r ** 2) <= 36).astype(float)
+    sens = np.zeros((Nx, Ny))
+    sens[10:Nx - 10, 34] = 1
+    sens[10, :35] = 1
+    sens[Nx - 11, :35] = 1
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    fwd = fe.save_to_disk_dict(kg, med, {"p0": p0}, {"mask": sens}, pml_size=[8, 8], axisymmetric=True)
+    rec = kw.run(fwd, record=("p",))["p"]
+    tr = fe.save_to_disk_dict(kg, med, {"p": rec.T[:, ::-1], "p_mask": sens, "p_mode": "dirichlet"},
+                              {"mask": np.ones((Nx, Ny))}, pml_size=[8, 8], axisymmetric=True)
+    inp, out = str(tmp_path / "tr.h5"), str(tmp_path / "tr_out.h5")
+    h5format.write_input(inp, tr)
+    res = h5format.run_binary(inp, out, ("p_final",))
+    assert res.returncode == 0, res.stderr
+    got = h5format.read_file(out)["p_final"].reshape(Ny, Nx)
+    ref = oracle.run(tr, record=("p_final",))["p_final"].reshape(Ny, Nx)
+    e = cases.rel_l2(got, ref)
+    print(f"    p_final rel L2 vs oracle {e:.2e}", flush=True)
+    assert e < 1e-4
+    p0s = np.asarray(fwd["p0_source_input"], dtype=np.float64).reshape(Ny, Nx)
+    inside = np.zeros((Ny, Nx), bool)
+    inside[:30, 15:Nx - 15] = True
+    cc = np.corrcoef(got[inside], p0s[inside])[0, 1]
+    print(f"    correlation with p0 inside the surface {cc:.4f}", flush=True)
+    assert cc > 0.99
 
 
 @pytest.mark.parametrize("how", ["abi", "binary"])
