@@ -331,3 +331,18 @@ def test_frequency_ref_source_front_end(kw):
     assert int(fu["u_source_mode"].ravel()[0]) == 1
     with pytest.raises(ValueError, match="maximum frequency"):
         cases.w_correction_plane_wave("p", variant="w", f0=8e6)
+
+
+def test_create_rejects_axisymmetric_misuse(kw):
+    """axisymmetric_flag: 2-D only, stokes or no absorption, no u_non_staggered
+    (checked before any device call); the radial PML may be as large as Ny."""
+    f = cases.case_axisym(Nx=32, Ny=16, Nt=4, source="p0", pml=4)
+    g = dict(f)
+    g["pml_y_size"] = np.array([[[17]]], dtype=np.uint64)
+    assert "PML" in str(_err(kw, g))
+    g = dict(f)
+    g["absorbing_flag"] = np.array([[[1]]], dtype=np.uint64)
+    g["alpha_coeff"] = np.array([[[0.5]]], dtype=np.float32)
+    g["alpha_power"] = np.array([[[1.5]]], dtype=np.float32)
+    assert "stokes" in str(_err(kw, g))
+    assert "u_non_staggered" in str(_err(kw, f, record=("p", "u_non_staggered")))
