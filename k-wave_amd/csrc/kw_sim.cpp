@@ -1312,7 +1312,8 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
       if (s->xl.nt) rows = s->g.Nx <= 96 ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256};
       // (y pass, last axis) workgroup sizes
       std::vector<std::pair<int, int>> pencils{{s->tly.nt, s->tll.nt}};
-      if (!s->g.kxm && (s->tly.nt || s->tll.nt))
+      // (axisymmetric grids: the radial pass replaces the pencils)
+      if (!s->g.kxm && !s->axisym && (s->tly.nt || s->tll.nt))
         pencils = {{128, 128}, {256, 256}, {256, 512}, {512, 512}};
       std::vector<int> planes{0};
       if (s->g.kxm)
