@@ -311,3 +311,23 @@ def test_axisymmetric_through_the_binary(kw, oracle, tmp_path):
     fa = fe.save_to_disk_dict(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0}, {"mask": sensor},
                               pml_size=8, axisymmetric=True)
     assert cases.rel_l2(sd["p"].T, oracle.run(fa, record=("p",))["p"]) < 1e-4
+
+
+def test_wrapper_options_accepted(kw, tmp_path):
+    """The options kspaceFirstOrder3DC.m passes (3DC.m:308-524: -t, -g, --verbose) and
+    deflate-compressed output (-c) leave the results unchanged."""
+    from kwave_amd import h5format
+
+    print("[gpu binary] -t / -g / --verbose / -c", flush=True)
+    f = cases.case_2d_absorbing_psource(N=(60, 48), Nt=30)
+    inp = str(tmp_path / "in.h5")
+    h5format.write_input(inp, f)
+    outs = {}
+    for tag, extra in (("plain", ()), ("opts", ("-t", "8", "--verbose", "2", "-c", "4"))):
+        out = str(tmp_path / f"{tag}.h5")
+        r = h5format.run_binary(inp, out, ("p", "p_max", "u_final"), device=0, extra=extra)
+        assert r.returncode == 0, r.stderr
+        outs[tag] = h5format.read_file(out)
+    assert outs["opts"].keys() == outs["plain"].keys()
+    for k in outs["plain"]:
+        np.testing.assert_array_equal(outs["opts"][k], outs["plain"][k])
