@@ -41,6 +41,21 @@ def test_2d_p0_heterogeneous_velocity_outputs(kw, oracle):
     compare(kw, oracle, f, ("p", "u", "u_max", "u_rms", "u_final", "p_max_all", "p_min_all"), "2d_p0_het")
 
 
+U_STATS = ("u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final", "p_min")
+
+
+@pytest.mark.parametrize("dim", ["2d", "3d_plane"])
+def test_every_velocity_statistic(kw, oracle, dim):
+    """u_min, u_max_all and u_min_all beside the other velocity outputs, with a
+    record start index past 1: every statistic, the whole-grid _all fields
+    included, runs over the recorded steps only (extractSensorData.m:61-297)."""
+    if dim == "2d":
+        f = cases.case_2d_p0_lossless(N=64, Nt=50, het=True)
+    else:
+        f = cases.case_3d(N=(32, 64, 64), Nt=30, source="uxyz")
+    compare(kw, oracle, f, U_STATS, f"u_stats_{dim}", start=7)
+
+
 @pytest.mark.parametrize("mode", ["additive", "additive-no-correction", "dirichlet"])
 def test_2d_absorbing_pressure_source(kw, oracle, mode):
     f = cases.case_2d_absorbing_psource(mode=mode)
