@@ -418,3 +418,44 @@ class CartesianSensor:
         """Interpolate recorded mask data (..., Nmask) -> (..., Ns) in float64."""
         r = np.asarray(rec, dtype=np.float64)
         return np.einsum("...sk,sk->...s", r[..., self.tri], self.bc)
+
+
+def fourier_shift(data, shift: float) -> np.ndarray:
+    """fourierShift.m along the last axis (the time axis of sensor data): the
+    band-limited interpolant of each series advanced by `shift` samples,
+    real(ifft(ifftshift(exp(i k shift)) * fft(data))), with the wavenumber
+    k = 2 pi m / N, m = -floor(N/2) .. ceil(N/2) - 1, the m = 0 entry set to an
+    exact 0 and the even-length Nyquist bin kept at -pi (fourierShift.m:81-112)."""
+    x = np.asarray(data, dtype=np.float64)
+    n = x.shape[-1]
+    m = np.arange(-(n // 2), n - n // 2, dtype=np.float64)
+    k = 2.0 * np.pi * m / n
+    k[n // 2] = 0.0
+    ph = np.fft.ifftshift(np.exp(1j * k * shift))
+    return np.real(np.fft.ifft(ph * np.fft.fft(x, axis=-1), axis=-1))
+
+
+def save_intensity(sensor_data, dim: int, record):
+    """kspaceFirstOrder_saveIntensity.m:42-104 on the sensor data the binary
+    returned (a dict, or a list of dicts, one per cuboid): u_non_staggered is
+    shifted forward by dt/2 in time (fourier_shift by 1/2) so it sits on p's
+    time grid, I_i = p * u_i, and I_i_avg is its mean over the time (last) axis.
+    Fields the record does not ask for (the time-varying intensity,
+    u_non_staggered, p) are dropped, as the MATLAB code removes them."""
+    if isinstance(sensor_data, (list, tuple)):
+        return [save_intensity(s, dim, record) for s in sensor_data]
+    out = dict(sensor_data)
+    p = np.asarray(out["p"], dtype=np.float64)
+    for a in "xyz"[:dim]:
+        I = p * fourier_shift(out[f"u{a}_non_staggered"], 0.5)
+        out[f"I{a}"] = I
+        if "I_avg" in record:
+            out[f"I{a}_avg"] = I.mean(axis=-1)
+    for a in "xyz"[:dim]:
+        if "u_non_staggered" not in record:
+            out.pop(f"u{a}_non_staggered")
+        if "I" not in record:
+            out.pop(f"I{a}")
+    if "p" not in record:
+        out.pop("p")
+    return out

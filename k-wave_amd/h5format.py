@@ -206,7 +206,7 @@ def sensor_data_from_output(o: dict, record, dim: int) -> dict:
     sd = {}
     for name in output_names(record, dim):
         a = o[name]
-        if name in ("p", "ux", "uy", "uz"):
+        if name in ("p", "ux", "uy", "uz") or name.endswith("_non_staggered"):
             sd[name] = a.reshape(a.shape[-2], a.shape[-1]).T          # [Ns, Nt_rec]
         elif name.endswith("_all") or name.endswith("_final"):
             sd[name] = a.transpose(2, 1, 0) if dim == 3 else a[0].T    # MATLAB [Nx, Ny(, Nz)]
@@ -219,16 +219,24 @@ def kspace_first_order(kgrid, medium, source, sensor, record=("p",), record_star
                        pml_alpha=2.0, smooth_p0=True, data_path=None, device=None, delete_data=True,
                        binary=BINARY_PATH, axisymmetric=False):
     """kspaceFirstOrder2DG/3DG (kspaceFirstOrderASC with axisymmetric=True): write
-    the input file, run the binary, read the output."""
+    the input file, run the binary, read the output.  'I' / 'I_avg' in the record
+    ask the binary for p and u_non_staggered and form the intensity on the host
+    (3DC.m:508-511, 786-790)."""
+    want_I = "I" in record or "I_avg" in record
+    record_bin = tuple(r for r in record if r not in ("I", "I_avg"))
+    if want_I:
+        record_bin += tuple(r for r in ("p", "u_non_staggered") if r not in record_bin)
     f = frontend.save_to_disk_dict(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha,
                                    smooth_p0=smooth_p0, axisymmetric=axisymmetric)
     d = data_path or tempfile.mkdtemp(prefix="kwave_")
     inp, out = os.path.join(d, "kwave_input.h5"), os.path.join(d, "kwave_output.h5")
     write_input(inp, f)
-    r = run_binary(inp, out, record, record_start_index, device, binary)
+    r = run_binary(inp, out, record_bin, record_start_index, device, binary)
     if r.returncode != 0:
         raise H5Error(f"kspaceFirstOrder-HIP failed ({r.returncode}): {r.stderr.strip()}")
-    sd = sensor_data_from_output(read_file(out), record, kgrid.dim)
+    sd = sensor_data_from_output(read_file(out), record_bin, kgrid.dim)
+    if want_I:
+        sd = frontend.save_intensity(sd, kgrid.dim, record)
     if delete_data:
         for p in (inp, out):
             if os.path.exists(p):

@@ -276,14 +276,14 @@ def fourier_shift(data, shift):
     return np.real(np.fft.ifft(np.fft.ifftshift(np.exp(1j * k * shift)) * np.fft.fft(data, axis=-1), axis=-1))
 
 
-def plane_wave_intensity():
-    """testing/unit/kspaceFirstOrder_check_plane_wave_intensity.m, 2-D x case: a
-    10 MHz plane wave of 10 W/cm^2 from a p source line; the time-averaged
-    intensity p .* fourierShift(ux_non_staggered, 1/2) (saveIntensity.m) over the
-    last two periods at a point across the grid must be 10 W/cm^2 within 1 %.
-    Returns (input dict, record_start_index, error_fn(p, ux_non_staggered) -> %)."""
+PLANE_WAVE_I0 = 10.0   # W/cm^2
+
+
+def plane_wave_intensity_objects():
+    """The kgrid, medium, source, sensor, PML options and record start index of
+    plane_wave_intensity(), for the kspaceFirstOrder2DG-style flow."""
     Nx, Nj, dx = 64, 32, 0.05e-3
-    c0, rho0, f0, I0 = 1500.0, 1000.0, 10e6, 10.0
+    c0, rho0, f0, I0 = 1500.0, 1000.0, 10e6, PLANE_WAVE_I0
     strength = np.sqrt(2 * I0 * 1e4 * rho0 * c0)
     ppp = 30
     dt = 1 / f0 / (2 * ppp)
@@ -298,9 +298,20 @@ def plane_wave_intensity():
     pm[20, :] = 1                                    # source_position = PML_size + 1
     sm = np.zeros((Nx, Nj))
     sm[Nx - 22, Nj // 2 - 1] = 1                     # sensor_position = Nx - PML_size - 1, Nj/2
-    f = fe.save_to_disk_dict(kg, {"sound_speed": c0, "density": rho0}, {"p": sig[None], "p_mask": pm},
-                             {"mask": sm}, pml_size=[20, 2], pml_alpha=[2.0, 0.0])
-    start = Nt - 2 * ppp + 1
+    pml = {"pml_size": [20, 2], "pml_alpha": [2.0, 0.0]}
+    return (kg, {"sound_speed": c0, "density": rho0}, {"p": sig[None], "p_mask": pm}, {"mask": sm}, pml,
+            Nt - 2 * ppp + 1)
+
+
+def plane_wave_intensity():
+    """testing/unit/kspaceFirstOrder_check_plane_wave_intensity.m, 2-D x case: a
+    10 MHz plane wave of 10 W/cm^2 from a p source line; the time-averaged
+    intensity p .* fourierShift(ux_non_staggered, 1/2) (saveIntensity.m) over the
+    last two periods at a point across the grid must be 10 W/cm^2 within 1 %.
+    Returns (input dict, record_start_index, error_fn(p, ux_non_staggered) -> %)."""
+    kg, medium, source, sensor, pml, start = plane_wave_intensity_objects()
+    f = fe.save_to_disk_dict(kg, medium, source, sensor, **pml)
+    I0 = PLANE_WAVE_I0
 
     def error_pct(p, ux_ns):
         p = np.asarray(p, dtype=np.float64).reshape(-1)

@@ -331,3 +331,24 @@ def test_wrapper_options_accepted(kw, tmp_path):
     assert outs["opts"].keys() == outs["plain"].keys()
     for k in outs["plain"]:
         np.testing.assert_array_equal(outs["opts"][k], outs["plain"][k])
+
+
+def test_intensity_through_the_front_end_flow(kw):
+    """sensor.record = {'I_avg'} / {'p', 'I'} in the kspaceFirstOrder2DG-style flow:
+    the binary records p and u_non_staggered (3DC.m:508-511), the host forms the
+    intensity (saveIntensity.m), and the plane wave's time-averaged intensity is
+    the source's 10 W/cm^2 within 1 % (kspaceFirstOrder_check_plane_wave_intensity.m)."""
+    from kwave_amd import h5format
+
+    kg, medium, source, sensor, pml, start = cases.plane_wave_intensity_objects()
+    print("[gpu binary] plane-wave intensity through kspace_first_order", flush=True)
+    sd = h5format.kspace_first_order(kg, medium, source, sensor, record=("I_avg",), record_start_index=start, **pml)
+    assert set(sd) == {"Ix_avg", "Iy_avg"}, sorted(sd)
+    err = 100 * abs(sd["Ix_avg"][0] / 1e4 - cases.PLANE_WAVE_I0) / cases.PLANE_WAVE_I0
+    print(f"    Ix_avg error {err:.3f} %", flush=True)
+    assert err < 1.0
+    assert abs(sd["Iy_avg"][0]) <= 1e-5 * abs(sd["Ix_avg"][0])
+    full = h5format.kspace_first_order(kg, medium, source, sensor, record=("p", "I"), record_start_index=start, **pml)
+    assert set(full) == {"p", "Ix", "Iy"}, sorted(full)
+    assert full["Ix"].shape == full["p"].shape == (1, kg.Nt - start + 1)
+    assert np.isclose(full["Ix"].mean(), sd["Ix_avg"][0], rtol=1e-12)
