@@ -174,12 +174,11 @@ int as_radial_tile(int Ny, size_t lds_max, size_t* lds) {
 void launch_as_radial(hipStream_t s, const Grid& g, const FftAxis& ax, const AsTables& T, const LastArgs& a, int tx,
                       size_t lds) {
   if (lds > 65536) {
-    static bool attr = false;
-    if (!attr) {
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_as_radial),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      attr = true;
-    }
+    // once per process; a function-local static initialiser is thread-safe
+    static const bool attr = ((void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_as_radial),
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                              true);
+    (void)attr;
   }
   // normalised inverse of the expanded transform: 2 / (4 Ny) from the odd-
   // frequency fold, 1 / Nx for the unnormalised x C2R that follows
