@@ -1,22 +1,33 @@
 """Benchmark: grid-point-updates/s of the k-space time loop on one MI355X.
 
-Workload (N=1): the 3-D 128^3 heterogeneous absorbing configuration (C2:
-16x16 ux additive source, mid-plane sensor recording p and p_max, Nt = 1200).
-A "step" is one time step of that simulation; the timed region runs K steps
-of a freshly created simulation from t = 1 (K = 1200 by default = the whole
-run), with inputs and operators already resident in HBM.  Warmup runs W steps
-on a separate instance first.
+Workload (N=1, default): the largest single-GPU configuration, 768^3
+heterogeneous absorbing three-layer slab (PML 20, 64x64 p additive patch,
+128x128 receiver plane recording p, CFL 0.3, Nt = 1500).  A "step" is one
+time step of that simulation; the timed region runs K steps of one resident
+simulation from t = 1 after W untimed warmup steps of a separate instance,
+with inputs and operators already resident in HBM.
 
-Also reported: the roofline of the dominant kernel (algorithmic bytes per
-launch / measured average launch time, per-kernel HIP events on the solver's
-stream), the whole-step algorithmic bandwidth, the FP64 CPU oracle timed on a
-bounded sample of the same workload, and the serving leg: a sample of the C5
-request stream run by the plain path (one simulation at a time) and by the
-serving loop (continuous batching), with throughput and p50/p99 latency for
-burst and Poisson arrivals.
+Also reported, on the same line:
+  roofline       the dominant kernel (largest share of the timed region):
+                 algorithmic bytes per launch / its average launch duration,
+                 measured with HIP events on the solver's stream over the
+                 timed region; `traffic` = its counter-measured HBM bytes per
+                 launch from the committed rocprofv3 summary
+  step_roofline  algorithmic bytes per grid point and step (212 B, +20 B per
+                 active additive source component) / step time
+  hbm_counters   counter-measured FETCH+WRITE bytes per step (committed
+                 summary, per kernel kind x launches per step) / step time
+  copy_ceiling   the best of several streaming device copies (>= 1 GiB)
+  cpu_baseline   the FP64 oracle on the host cores, a bounded sample of the
+                 same workload (one step), extrapolated to Nt
+  c2             the 128^3 configuration (plane-pass layout) as a second line
+  serving        a sample of the serving stream: plain path (one simulation at
+                 a time) vs the serving loop, burst and Poisson arrivals at
+                 several loads, p50/p99 latency, and each mode's relative L2 /
+                 max-abs error against the FP64 oracle
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload c2|c3|c4]
-(--workload c3/c4 times the large configurations, without the serving leg or the CPU baseline)
+Usage: python bench.py [--gpus 1] [--workload c4|c2|c3] [--steps K] [--warmup W]
+                       [--no-cpu] [--serving-requests M] [--serving-rates 0.5,0.9,1.5]
 """
 
 from __future__ import annotations
@@ -34,16 +45,27 @@ sys.path.insert(0, ROOT)
 from __graft_entry__ import import_pkg  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+BASE_BYTES_PER_POINT = {"c2": 212.0, "c3": 208.0, "c4": 212.0}  # algorithmic model, per step
+SOURCE_BYTES_PER_POINT = 20.0  # per additive source component while its signal lasts
 
 
-def kernel_bytes(f: dict, kind: str, launches_per_step: dict) -> float:
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _dims(f):
+    return tuple(int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz"))
+
+
+def kernel_bytes(f: dict, kind: str) -> float:
     """Algorithmic HBM bytes of one launch of a kernel kind at this workload.
 
     Counts each distinct array the kernel must read or write once: real
     fields 4 B/point, X-domain half spectra (Nx/2+1)*Ny*Nz*8 B, spectral
-    tables 4 B per half-spectrum point; scalars and 1-D vectors 0.
+    tables 4 B per half-spectrum point; scalars and 1-D vectors 0.  The
+    spectral passes are averaged over one step's launches of that kind.
     """
-    Nx, Ny, Nz = (int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz"))
+    Nx, Ny, Nz = _dims(f)
     N = Nx * Ny * Nz
     spec = (Nx // 2 + 1) * Ny * Nz * 8
     tab = (Nx // 2 + 1) * Ny * Nz * 4
@@ -60,21 +82,37 @@ def kernel_bytes(f: dict, kind: str, launches_per_step: dict) -> float:
     if kind == "x_pressure":    # 2 G in, sum(rho), c0, tau, eta, Pk out
         fields = 1 + (1 if het("c0") else 0) + (2 if (het("c0") or het("alpha_coeff")) else 0)
         return 2 * spec + fields * fld + spec
-    if kind in ("last_axis", "yz_plane"):  # averaged over the step's launches: in + out spectra + tables
+    if kind in ("last_axis", "yz_plane"):  # in + out spectra + tables, averaged over the step's launches
         # gradient 1->dim (+kappa), divergence dim->dim (+kappa), absorption 2->2 (+nabla1, nabla2)
         launches = [(1, dim, 1), (dim, dim, 1)] + ([(2, 2, 2)] if absorbing == 1 else [])
-        tot = sum((i + o) * spec + t * tab for i, o, t in launches)
-        return tot / len(launches)
-    if kind == "y_pass":        # averaged: read + write of the fields in each launch
-        nf = [1, dim, dim, dim, dim] + ([2, 2] if absorbing == 1 else [])
+        return sum((i + o) * spec + t * tab for i, o, t in launches) / len(launches)
+    if kind == "y_pass":        # read + write of the fields of each launch: the inputs and the
+        # outputs of each spectral operator (gradient 1, dim; divergence dim, dim; absorption 2, 2)
+        nf = [1, dim, dim, dim] + ([2, 2] if absorbing == 1 else [])
         return sum(2 * n * spec for n in nf) / len(nf)
     return 0.0
 
 
-def profiled_traffic(workload: str, kind: str):
-    """HBM bytes per launch of a kernel kind at this workload, from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/<workload>_*.md, calibrated
-    against a plain copy); None when no summary covers it."""
+def step_bytes_per_point(f: dict, wl: str, t_first: int, t_last: int) -> float:
+    """Mean algorithmic bytes per grid point and step over steps t_first..t_last:
+    the configuration's base figure plus 20 B per additive source component
+    whose signal is still running."""
+    comps = []
+    for k in ("p", "ux", "uy", "uz"):
+        flag = int(np.asarray(f.get(k + "_source_flag", 0)).reshape(-1)[0])
+        mode = int(np.asarray(f.get(("p" if k == "p" else "u") + "_source_mode", 0)).reshape(-1)[0])
+        if flag and mode == 2:
+            comps.append(flag)
+    steps = np.arange(t_first, t_last + 1)
+    active = sum((steps <= fl).astype(float) for fl in comps) if comps else np.zeros(len(steps))
+    return float(np.mean(BASE_BYTES_PER_POINT[wl] + SOURCE_BYTES_PER_POINT * active))
+
+
+def profiled_traffic(workload: str):
+    """Counter-measured HBM bytes per launch by kernel kind at this workload,
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/<workload>_*.md, calibrated against a plain copy); ({}, None)
+    when no summary covers it."""
     import glob
     import re
 
@@ -82,14 +120,204 @@ def profiled_traffic(workload: str, kind: str):
         m = re.search(r"```json\s*(\{.*?\})\s*```", open(path).read(), re.S)
         if m:
             t = json.loads(m.group(1)).get("traffic_bytes_per_launch", {})
-            if kind in t:
-                return float(t[kind]), os.path.relpath(path, ROOT)
-    return None, None
+            if t:
+                return {k: float(v) for k, v in t.items()}, os.path.relpath(path, ROOT)
+    return {}, None
 
 
 def _updates(f: dict) -> int:
     return int(np.prod([int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz", "Nt")]))
 
+
+def cpu_info() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "cores": len(os.sched_getaffinity(0))}
+
+
+def mem_available() -> int:
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemAvailable:"):
+                return int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    return 0
+
+
+# ---------------------------------------------------------------------------
+# device copy: counter calibration kernel and the measured copy ceiling
+# ---------------------------------------------------------------------------
+
+def copy_legs(kw, calib_mib: int, ceiling_gib: int) -> dict:
+    out = {}
+    if calib_mib:
+        ms = kw.native.diag_copy(calib_mib << 20, variant=0)
+        out["calibration_copy_GBs"] = round(2 * (calib_mib << 20) / (ms * 1e-3) / 1e9, 1)
+        log(f"calibration copy (variant 0): {calib_mib} MiB in {ms:.3f} ms ({out['calibration_copy_GBs']} GB/s)")
+    if ceiling_gib:
+        nbytes = ceiling_gib << 30
+        best, sweep = None, []
+        for variant in (0, 1, 2, 3, 4):
+            for blocks in (2048, 8192, 32768):
+                ms = kw.native.diag_copy(nbytes, variant=variant, blocks=blocks)
+                gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+                sweep.append((variant, blocks, round(gbs, 1)))
+                if best is None or gbs > best[2]:
+                    best = (variant, blocks, gbs)
+        out["copy_ceiling"] = {"GBs": round(best[2], 1), "frac_of_peak": round(best[2] / HBM_PEAK_GBS, 4),
+                               "variant": best[0], "blocks": best[1], "bytes": nbytes,
+                               "sweep_variant_blocks_GBs": sweep}
+        log(f"copy ceiling: {best[2]:.0f} GB/s (variant {best[0]}, {best[1]} blocks) over {ceiling_gib} GiB")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# one configuration on one simulation
+# ---------------------------------------------------------------------------
+
+def measure_workload(kw, wl: str, K: int, W: int, keep_inputs: bool = False) -> tuple[dict, dict | None]:
+    """Times K steps of the workload from t = 1; returns (line fields, inputs)."""
+    from kwave_amd import configs
+
+    t_gen = time.perf_counter()
+    if wl == "c2":
+        f, rec, desc = configs.c2(Nt=max(K, W, 1)), configs.C2_RECORD, configs.C2_DESCRIPTION
+        data = "synthetic (seeded Gaussian random medium, default_rng(1002))"
+        nt_full = 1200
+    elif wl == "c3":
+        f, rec, desc = configs.c3(Nt=max(K, W, 1)), configs.C3_RECORD, configs.C3_DESCRIPTION
+        data = "synthetic (homogeneous medium, makeBowl source)"
+        nt_full = 2000
+    else:
+        f, rec, desc = configs.c4(Nt=max(K, W, 1), log=True), configs.C4_RECORD, configs.C4_DESCRIPTION
+        data = "synthetic (seeded three-layer slab with Gaussian perturbation, default_rng(1004))"
+        nt_full = 1500
+    log(f"{wl}: inputs generated in {time.perf_counter() - t_gen:.1f} s")
+    Nx, Ny, Nz = _dims(f)
+    N = Nx * Ny * Nz
+
+    def timed(sim, steps):
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.run(steps)
+        sim.sync()
+        return time.perf_counter() - t0
+
+    if W > 0:  # warmup on a separate instance (code objects, caches, allocator pools)
+        with kw.Simulation(f, record=rec) as sim:
+            timed(sim, W)
+        log(f"{wl}: {W} warmup steps done")
+    with kw.Simulation(f, record=rec) as sim:
+        mem = sim.memory()
+        log(f"{wl}: simulation created ({mem['allocated'] / 2**30:.2f} GiB on the device)")
+        # per-kernel HIP events on the solver stream over the timed region; on
+        # small grids (launch-bound) the event pairs perturb the step, so they
+        # time a second run of the same steps instead
+        in_region = N >= 64 * 2**20
+        sim.set_profiling(in_region)
+        elapsed = timed(sim, K)
+        prof = sim.profile() if in_region else None
+        chk = sim.fetch(rec[-1])
+    if not np.all(np.isfinite(chk)):
+        raise RuntimeError(f"non-finite {rec[-1]} in the benchmark run")
+    del chk
+    if prof is None:
+        with kw.Simulation(f, record=rec) as sim:
+            sim.set_profiling(True)
+            sim.run(K)
+            sim.sync()
+            prof = sim.profile()
+    log(f"{wl}: {K} timed steps in {elapsed:.3f} s ({1e3 * elapsed / K:.3f} ms/step)")
+
+    ms_per_step = 1e3 * elapsed / K
+    kinds = {k: v for k, v in prof.items() if v[1] > 0}
+    total_ms = sum(v[0] for v in kinds.values())
+    dom = max(kinds, key=lambda k: kinds[k][0])
+    dom_ms, dom_n = kinds[dom]
+    dom_avg_s = dom_ms / dom_n / 1e3
+    dom_bytes = kernel_bytes(f, dom)
+    achieved = dom_bytes / dom_avg_s / 1e9
+    per_kernel = {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1],
+                      "share": round(v[0] / total_ms, 3),
+                      "algorithmic_bytes_per_launch": kernel_bytes(f, k),
+                      "algorithmic_GBs": round(kernel_bytes(f, k) / (v[0] / v[1] / 1e3) / 1e9, 1)}
+                  for k, v in kinds.items()}
+    traffic, traffic_src = profiled_traffic(wl)
+    bpp = step_bytes_per_point(f, wl, 1, K)
+    step_GBs = bpp * N / (ms_per_step / 1e3) / 1e9
+    line = {
+        "value": N * K / elapsed,
+        "ms_per_step": ms_per_step,
+        "data": data,
+        "config": {"workload": wl.upper() + " " + desc, "grid": [Nx, Ny, Nz], "Nt": nt_full,
+                   "timed_steps": [1, K], "record": list(rec)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get(dom), "traffic_source": traffic_src,
+                     "bytes_per_launch": dom_bytes, "avg_launch_us": round(dom_avg_s * 1e6, 2),
+                     "timing": "HIP events on the solver stream over the timed region" if in_region
+                               else "HIP events on the solver stream, a second run of the same steps"},
+        "step_roofline": {"bytes_per_point": round(bpp, 2), "achieved_GBs": round(step_GBs, 1),
+                          "frac": round(step_GBs / HBM_PEAK_GBS, 4)},
+        "kernels": per_kernel,
+    }
+    if traffic and all(k in traffic for k in kinds):
+        step_traffic = sum(traffic[k] * kinds[k][1] / K for k in kinds)
+        cnt_GBs = step_traffic / (ms_per_step / 1e3) / 1e9
+        line["hbm_counters"] = {"bytes_per_step": step_traffic, "achieved_GBs": round(cnt_GBs, 1),
+                                "frac": round(cnt_GBs / HBM_PEAK_GBS, 4),
+                                "vs_algorithmic": round(step_traffic / (bpp * N), 3), "source": traffic_src}
+    return line, (f if keep_inputs else None)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the FP64 oracle on a bounded sample of the same workload
+# ---------------------------------------------------------------------------
+
+def cpu_baseline(f: dict, rec, steps: int, nt_full: int) -> dict:
+    import scipy
+
+    from oracle import kwave_oracle  # CPU baseline only
+
+    Nx, Ny, Nz = _dims(f)
+    N = Nx * Ny * Nz
+    info = cpu_info()
+    need = 320 * N  # FP64 oracle working set, measured ~300 B per grid point at 256^3
+    avail = mem_available()
+    if avail and need > 0.85 * avail:
+        return {"value": None, "unit": "grid-point-updates/s", "cores": info["cores"], "kind": "port",
+                "sample": f"skipped: the FP64 oracle needs ~{need / 2**30:.0f} GiB, {avail / 2**30:.0f} GiB available"}
+    tm = {}
+    t0 = time.perf_counter()
+    kwave_oracle.run(f, record=rec, nsteps=steps, timings=tm)
+    wall = time.perf_counter() - t0
+    t_act = float(np.mean(tm["step"]))
+    t_src = float(np.mean(tm["source"]))
+    t_steady = max(t_act - t_src, 1e-9)
+    flags = [int(np.asarray(f.get(k + "_source_flag", 0)).reshape(-1)[0]) for k in ("p", "ux", "uy", "uz")]
+    n_src = min(nt_full, max(flags))
+    T = n_src * t_act + (nt_full - n_src) * t_steady
+    return {"value": N * nt_full / T, "unit": "grid-point-updates/s", "cores": info["cores"], "kind": "port",
+            "extrapolated": True,
+            "sample": (f"FP64 numpy {np.__version__} / scipy {scipy.__version__} (scipy.fft pocketfft, "
+                       f"workers={info['cores']}, slab-parallel elementwise) oracle on {info['cpu_model']}: "
+                       f"{steps} time step(s) of the same {Nx}x{Ny}x{Nz} workload with the source active, "
+                       f"{t_act:.1f} s/step of which {t_src:.2f} s in the source branch; extrapolated to "
+                       f"Nt={nt_full} as {n_src} source-active steps + {nt_full - n_src} steady-state steps "
+                       f"of {t_steady:.1f} s (oracle call {wall:.0f} s incl. operator set-up)"),
+            "step_s": round(t_act, 3), "steady_step_s": round(t_steady, 3), "cpu_model": info["cpu_model"]}
+
+
+# ---------------------------------------------------------------------------
+# serving leg
+# ---------------------------------------------------------------------------
 
 def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int, autotune: bool = False):
     """Submit reqs at the given arrival offsets (s) to a fresh server; wait for all."""
@@ -120,32 +348,66 @@ def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int, autotune: b
     return res, outs
 
 
-def serving_leg(kw, M: int, rates, max_active: int, quantum: int) -> dict:
-    """C5 serving stream: plain (one simulation at a time) vs served (continuous
-    batching over streams) on the same requests, burst and Poisson arrivals."""
+def _err(got, ref):
+    g = np.asarray(got, dtype=np.float64)
+    r = np.asarray(ref, dtype=np.float64)
+    nr = np.linalg.norm(r)
+    return float(np.linalg.norm(g - r) / (nr if nr > 0 else 1.0)), float(np.max(np.abs(g - r)))
+
+
+def serving_leg(kw, M: int, rates, max_active: int, quantum: int, accuracy_requests: int) -> dict:
+    """Serving stream sample: plain (one simulation at a time) vs served
+    (continuous batching over streams) on the same requests, burst and Poisson
+    arrivals; each mode's error against the FP64 oracle on the smallest requests."""
     import concurrent.futures as cf
+
     from kwave_amd import configs
 
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(8) as ex:
         reqs = list(ex.map(configs.c5_request, range(M)))
-    gen_s = time.perf_counter() - t0
-    print(f"serving: generated {M} requests in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    log(f"serving: generated {M} requests in {time.perf_counter() - t0:.1f} s")
     kw.run(configs.c5_request(0, grids=((32, 32, 32),), nt_range=(8, 8)))  # code objects loaded
     burst = [0.0] * M
     plain, plain_out = _serve_stream(kw, reqs, burst, 1, quantum)
-    print(f"serving: plain burst {plain}", file=sys.stderr, flush=True)
+    log(f"serving: plain burst {plain}")
     served, served_out = _serve_stream(kw, reqs, burst, max_active, quantum, autotune=True)
-    print(f"serving: served burst {served}", file=sys.stderr, flush=True)
+    log(f"serving: served burst {served}")
     # served requests run autotuned launch shapes: equal to the plain path to FP32 rounding
-    rel = max(float(np.linalg.norm(b.astype(np.float64) - a) / max(np.linalg.norm(a.astype(np.float64)), 1e-30))
-              for a, b in zip(plain_out, served_out))
-    del plain_out, served_out
-    out = {"requests": M, "sample": f"c5_request(0..{M - 1}) of the seeded serving stream",
+    rel = max(_err(b, a)[0] for a, b in zip(plain_out, served_out))
+    grids = {}
+    for f in reqs:
+        g = "x".join(str(n) for n in _dims(f))
+        grids[g] = grids.get(g, 0) + 1
+    out = {"requests": M, "sample": f"c5_request(0..{M - 1}) of the seeded serving stream", "grids": grids,
            "updates": sum(_updates(f) for f in reqs), "steps_per_quantum": quantum,
            "burst": {"plain": plain, "served": served,
                      "speedup": round(served["updates_per_s"] / plain["updates_per_s"], 3)},
-           "max_rel_l2_vs_plain": rel, "matches_plain": bool(rel <= 1e-5), "poisson": []}
+           "max_rel_l2_served_vs_plain": rel, "matches_plain": bool(rel <= 1e-5), "poisson": []}
+    if accuracy_requests > 0:
+        from oracle import kwave_oracle  # accuracy check only
+
+        order = sorted(range(M), key=lambda i: (int(np.prod(_dims(reqs[i]))), _updates(reqs[i])))[:accuracy_requests]
+        acc = {"plain": {"max_rel_l2": 0.0, "max_abs": 0.0}, "served": {"max_rel_l2": 0.0, "max_abs": 0.0},
+               "requests": []}
+        for i in order:
+            t1 = time.perf_counter()
+            ref = kwave_oracle.run(reqs[i], record=("p",))["p"]
+            e_p = _err(plain_out[i], ref)
+            e_s = _err(served_out[i], ref)
+            for mode, e in (("plain", e_p), ("served", e_s)):
+                acc[mode]["max_rel_l2"] = max(acc[mode]["max_rel_l2"], e[0])
+                acc[mode]["max_abs"] = max(acc[mode]["max_abs"], e[1])
+            acc["requests"].append({"request": i, "grid": list(_dims(reqs[i])),
+                                    "Nt": int(np.asarray(reqs[i]["Nt"]).reshape(-1)[0]),
+                                    "peak_abs_p": float(np.max(np.abs(ref))),
+                                    "plain_rel_l2": e_p[0], "plain_max_abs": e_p[1],
+                                    "served_rel_l2": e_s[0], "served_max_abs": e_s[1]})
+            log(f"serving: request {i} vs FP64 oracle ({time.perf_counter() - t1:.1f} s): plain {e_p}, served {e_s}")
+        acc["tolerance_rel_l2"] = 1e-4
+        acc["within_tolerance"] = bool(max(acc["plain"]["max_rel_l2"], acc["served"]["max_rel_l2"]) <= 1e-4)
+        out["accuracy_vs_oracle"] = acc
+    del plain_out, served_out
     capacity = M / plain["makespan_s"]  # plain-path requests per second
     rng = np.random.default_rng(5999)
     gaps = rng.exponential(1.0, M)
@@ -154,29 +416,36 @@ def serving_leg(kw, M: int, rates, max_active: int, quantum: int) -> dict:
         arr = list(np.cumsum(gaps / lam) - gaps[0] / lam)
         p, _ = _serve_stream(kw, reqs, arr, 1, quantum)
         s, _ = _serve_stream(kw, reqs, arr, max_active, quantum, autotune=True)
-        out["poisson"].append({"rate_per_s": round(lam, 3), "load_vs_plain_capacity": r, "plain": p, "served": s})
-        print(f"serving: poisson load {r}: plain p50/p99 {p['p50_latency_s']}/{p['p99_latency_s']} s, "
-              f"served {s['p50_latency_s']}/{s['p99_latency_s']} s", file=sys.stderr, flush=True)
+        out["poisson"].append({"rate_per_s": round(lam, 3), "load_vs_plain_capacity": r, "plain": p, "served": s,
+                               "throughput_ratio": round(s["updates_per_s"] / p["updates_per_s"], 3)})
+        log(f"serving: poisson load {r}: plain p50/p99 {p['p50_latency_s']}/{p['p99_latency_s']} s, "
+            f"served {s['p50_latency_s']}/{s['p99_latency_s']} s")
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2",
-                    help="c2 (default): the 128^3 headline workload; c3/c4: the large configurations")
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 1200 for c2, 100 otherwise)")
-    ap.add_argument("--warmup", type=int, default=None, help="warmup steps (default: 100 for c2, 10 otherwise)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c4",
+                    help="c4 (default): 768^3, the largest single-GPU configuration; c2 128^3; c3 256x256x512")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 100; 1200 for c2)")
+    ap.add_argument("--warmup", type=int, default=None, help="warmup steps (default: 10; 100 for c2)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="oracle steps of the CPU baseline sample (default: 1; 120 for c2)")
     ap.add_argument("--calib-copy-mib", type=int, default=1024,
-                    help="also run one plain device copy of this many MiB (counter calibration)")
-    ap.add_argument("--serving-requests", type=int, default=24,
-                    help="C5 serving-stream sample size (0 skips the serving leg)")
-    ap.add_argument("--serving-rates", default="0.9",
+                    help="one plain device copy of this many MiB (counter calibration); 0 skips")
+    ap.add_argument("--ceiling-copy-gib", type=int, default=4,
+                    help="streaming-copy sweep over this many GiB (the copy ceiling); 0 skips")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 128^3 second line")
+    ap.add_argument("--serving-requests", type=int, default=48,
+                    help="serving-stream sample size (0 skips the serving leg)")
+    ap.add_argument("--serving-rates", default="0.5,0.9,1.5",
                     help="Poisson arrival rates as fractions of the plain path's capacity")
     ap.add_argument("--serving-max-active", type=int, default=8)
     ap.add_argument("--serving-quantum", type=int, default=16)
+    ap.add_argument("--accuracy-requests", type=int, default=2,
+                    help="smallest serving requests checked against the FP64 oracle (both modes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -184,131 +453,58 @@ def main():
         sys.exit(2)
 
     kw = import_pkg()
-    from kwave_amd import configs
-
-    copy_gbs = None
-    if args.calib_copy_mib:
-        ms = kw.native.diag_copy(args.calib_copy_mib << 20)
-        copy_gbs = 2 * (args.calib_copy_mib << 20) / (ms * 1e-3) / 1e9
-        print(f"calibration copy: {args.calib_copy_mib} MiB in {ms:.3f} ms ({copy_gbs:.0f} GB/s read+write)",
-              file=sys.stderr, flush=True)
-
     wl = args.workload
     K = args.steps if args.steps is not None else (1200 if wl == "c2" else 100)
     W = args.warmup if args.warmup is not None else (100 if wl == "c2" else 10)
-    prof_steps = min(K, 200 if wl == "c2" else 50)
-    t_gen = time.perf_counter()
-    if wl == "c2":
-        f, rec, desc, bpp = configs.c2(Nt=max(K, W, 1)), configs.C2_RECORD, configs.C2_DESCRIPTION, 212.0
-        data = "synthetic (seeded Gaussian random medium, default_rng(1002))"
-    elif wl == "c3":
-        f, rec, desc, bpp = (configs.c3(Nt=W + K + prof_steps), configs.C3_RECORD, configs.C3_DESCRIPTION,
-                             208.0)
-        data = "synthetic (homogeneous medium, makeBowl source)"
-    else:
-        f, rec, desc, bpp = (configs.c4(Nt=W + K + prof_steps, log=True), configs.C4_RECORD,
-                             configs.C4_DESCRIPTION, 212.0)
-        data = "synthetic (seeded three-layer slab with Gaussian perturbation, default_rng(1004))"
-    print(f"{wl}: inputs generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
-    Nx, Ny, Nz = (int(np.asarray(f[k]).reshape(-1)[0]) for k in ("Nx", "Ny", "Nz"))
-    N = Nx * Ny * Nz
 
-    def timed(sim, steps):
-        sim.sync()
-        t0 = time.perf_counter()
-        sim.run(steps)
-        sim.sync()
-        return time.perf_counter() - t0
-
-    if wl == "c2":
-        # warmup on a separate instance; timed region: K steps of a fresh simulation from t = 1
-        if W > 0:
-            with kw.Simulation(f, record=rec) as sim:
-                sim.run(W)
-                sim.sync()
-        with kw.Simulation(f, record=rec) as sim:
-            elapsed = timed(sim, K)
-            chk = sim.fetch(rec[-1])
-        # per-kernel timing (HIP events around every launch, on the solver stream)
-        with kw.Simulation(f, record=rec) as sim:
-            sim.set_profiling(True)
-            sim.run(prof_steps)
-            sim.sync()
-            prof = sim.profile()
-    else:
-        # one resident simulation (tens of GB): W warmup steps, K timed, then profiled steps
-        with kw.Simulation(f, record=rec) as sim:
-            print(f"{wl}: simulation created ({kw.native.footprint_bytes(f, rec) / 2**30:.1f} GiB)",
-                  file=sys.stderr, flush=True)
-            if W > 0:
-                timed(sim, W)
-            elapsed = timed(sim, K)
-            print(f"{wl}: {K} timed steps in {elapsed:.2f} s", file=sys.stderr, flush=True)
-            sim.set_profiling(True)
-            sim.run(prof_steps)
-            sim.sync()
-            prof = sim.profile()
-            chk = sim.fetch(rec[-1])
-    if not np.all(np.isfinite(chk)):
-        raise RuntimeError(f"non-finite {rec[-1]} in the benchmark run")
-    del chk
-    value = N * K / elapsed
-    ms_per_step = 1e3 * elapsed / K
-    kinds = {k: v for k, v in prof.items() if v[1] > 0}
-    dom = max(kinds, key=lambda k: kinds[k][0])
-    dom_ms, dom_n = kinds[dom]
-    dom_avg_s = dom_ms / dom_n / 1e3
-    dom_bytes = kernel_bytes(f, dom, {})
-    achieved = dom_bytes / dom_avg_s / 1e9
-    per_kernel = {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1],
-                      "share": round(v[0] / sum(x[0] for x in kinds.values()), 3)} for k, v in kinds.items()}
-
-    traffic, traffic_src = profiled_traffic(wl, dom)
-    step_bytes = bpp * N  # algorithmic bytes per time step of this workload
+    copies = copy_legs(kw, args.calib_copy_mib, args.ceiling_copy_gib)
+    line, f = measure_workload(kw, wl, K, W, keep_inputs=not args.no_cpu)
     out = {
         "metric": "grid-point-updates/s",
-        "value": value,
+        "value": line["value"],
         "unit": "grid-point-updates/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": line["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": data,
-        "config": {"workload": wl.upper() + " " + desc, "grid": [Nx, Ny, Nz], "Nt": K,
-                   "record": list(rec)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": traffic_src, "bytes_per_launch": dom_bytes,
-                     "avg_launch_us": round(dom_avg_s * 1e6, 2)},
-        "step_roofline": {"bytes_per_point": bpp, "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
-                          "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "kernels": per_kernel,
+        "data": line["data"],
+        "config": line["config"],
+        "roofline": line["roofline"],
+        "step_roofline": line["step_roofline"],
     }
-    if copy_gbs:
-        out["copy_GBs"] = round(copy_gbs, 1)
+    for k in ("hbm_counters", "kernels"):
+        if k in line:
+            out[k] = line[k]
+    out.update(copies)
+    if "copy_ceiling" in copies:
+        c = copies["copy_ceiling"]["GBs"]
+        out["step_roofline"]["frac_of_copy_ceiling"] = round(out["step_roofline"]["achieved_GBs"] / c, 4)
+        if "hbm_counters" in out:
+            out["hbm_counters"]["frac_of_copy_ceiling"] = round(out["hbm_counters"]["achieved_GBs"] / c, 4)
 
-    if args.serving_requests > 0 and wl == "c2":
+    if wl != "c2" and not args.no_c2:
+        c2, _ = measure_workload(kw, "c2", 1200, 100)
+        out["c2"] = {k: c2[k] for k in ("value", "ms_per_step", "roofline", "step_roofline", "kernels",
+                                        "hbm_counters") if k in c2}
+        out["c2"]["workload"] = c2["config"]["workload"]
+
+    if args.serving_requests > 0:
         rates = [float(x) for x in args.serving_rates.split(",") if x]
         out["serving"] = serving_leg(kw, args.serving_requests, rates, args.serving_max_active,
-                                     args.serving_quantum)
+                                     args.serving_quantum, args.accuracy_requests)
 
-    if not args.no_cpu and wl == "c2":
-        from oracle import kwave_oracle  # CPU baseline only
+    if not args.no_cpu:
+        from kwave_amd import configs
 
-        cs = args.cpu_steps
-        t0 = time.perf_counter()
-        kwave_oracle.run(f, record=rec, nsteps=cs)
-        tc = time.perf_counter() - t0
-        cores = len(os.sched_getaffinity(0))
-        out["cpu_baseline"] = {"value": N * cs / tc, "unit": "grid-point-updates/s", "cores": cores,
-                               "kind": "port",
-                               "sample": f"FP64 numpy/scipy.fft oracle, first {cs} time steps of the same "
-                                         f"workload (source active), scipy.fft workers={cores}, "
-                                         f"{tc:.1f} s"}
+        cs = args.cpu_steps if args.cpu_steps is not None else (120 if wl == "c2" else 1)
+        rec = {"c2": configs.C2_RECORD, "c3": configs.C3_RECORD, "c4": configs.C4_RECORD}[wl]
+        log(f"cpu baseline: FP64 oracle, {cs} step(s) of {wl}")
+        out["cpu_baseline"] = cpu_baseline(f, rec, cs, out["config"]["Nt"])
+        log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
 

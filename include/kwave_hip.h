@@ -203,15 +203,28 @@ int kw_sim_profile_get(kw_sim* sim, const char** names, double* total_ms, uint64
  * No counterpart in the reference (its binaries have fixed launch shapes). */
 kw_status kw_sim_autotune(kw_sim* sim, int trial_steps, int* row_nt, int* pencil_nt, int* plane_nt);
 
-/* Device memory (bytes) a description would need; 0 on invalid input.
- * Used for HBM-footprint admission by the serving loop. */
+/* Device memory (bytes) a description would need at its peak during
+ * kw_sim_create and after it; 0 on invalid input.  Follows kw_sim_create's
+ * allocation sequence.  Used for HBM-footprint admission by the serving loop. */
 uint64_t kw_sim_footprint_bytes(const kw_sim_desc* desc);
+
+/* Device bytes a handle holds now and held at most since kw_sim_create
+ * (the figures kw_sim_footprint_bytes predicts).  No counterpart in the
+ * reference. */
+kw_status kw_sim_memory(const kw_sim* sim, uint64_t* allocated_bytes, uint64_t* peak_bytes);
 
 /* Diagnostic with no counterpart in the reference: plain device copies of
  * `bytes` bytes (one untimed, then ten timed with HIP events; *ms is the mean
  * per copy).  Gives the FETCH_SIZE/WRITE_SIZE counters a kernel of known
  * traffic for calibration, and the copy bandwidth a practical ceiling. */
 kw_status kw_diag_copy(uint64_t bytes, int device, double* ms);
+
+/* The same with a choice of copy kernel, for the measured copy ceiling:
+ * variant 0 is kw_diag_copy's kernel (one 16-byte load per thread per
+ * iteration); 1 and 2 keep 4 and 8 16-byte loads in flight per thread; 3 and
+ * 4 are 1 and 2 with non-temporal loads and stores.  blocks: number of
+ * 256-thread workgroups (0: 2048). */
+kw_status kw_diag_copy_ex(uint64_t bytes, int device, int variant, int blocks, double* ms);
 
 /* ---------------------------------------------------------------------------
  * Serving loop.  The reference runs one simulation per process launch
@@ -223,7 +236,11 @@ kw_status kw_diag_copy(uint64_t bytes, int device, double* ms);
  * concurrently (each on its own stream), and their outputs are copied to
  * host memory when they finish, after which their device memory is freed and
  * the next request can be admitted.  A request's results are the same as
- * those of kw_sim_create + kw_sim_run(Nt) + kw_sim_fetch on its own.
+ * those of kw_sim_create + kw_sim_run(Nt) + kw_sim_fetch on its own, bit for
+ * bit -- except with kw_server_config.autotune set: a tuned request runs the
+ * workgroup sizes kw_sim_autotune picked for its shape (the same for every
+ * request of that shape in the process), which may round differently from
+ * the default sizes (FP32 rounding level).
  * ------------------------------------------------------------------------- */
 #define KW_ERR_TIMEOUT ((kw_status)6) /* kw_server_wait: not finished in time */
 

@@ -10,6 +10,8 @@
 // Statement references are to k-Wave/kspaceFirstOrder3D.m.
 #include <math.h>
 
+#include <algorithm>
+
 #include "kw_kernels.hpp"
 
 namespace kw {
@@ -907,6 +909,65 @@ __global__ void k_copy(const float4* __restrict__ in, float4* __restrict__ out, 
 void launch_copy(hipStream_t s, const void* in, void* out, size_t bytes) {
   const long long n4 = (long long)(bytes / 16);
   hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, (const float4*)in, (float4*)out, n4);
+}
+
+// Streaming copy with U 16-byte loads in flight per thread (all loads of an
+// iteration issue before its stores), optionally with non-temporal hints:
+// the measured device copy ceiling.
+typedef float kw_f4 __attribute__((ext_vector_type(4)));
+template <int U, bool NTMP>
+__global__ void __launch_bounds__(256) k_copy_u(const kw_f4* __restrict__ in, kw_f4* __restrict__ out, long long n4) {
+  const long long stride = (long long)gridDim.x * 256 * U;
+  for (long long base = (long long)blockIdx.x * 256 * U + threadIdx.x; base < n4; base += stride) {
+    kw_f4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = base + (long long)u * 256;
+      if (i < n4) v[u] = NTMP ? __builtin_nontemporal_load(&in[i]) : in[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = base + (long long)u * 256;
+      if (i < n4) {
+        if (NTMP)
+          __builtin_nontemporal_store(v[u], &out[i]);
+        else
+          out[i] = v[u];
+      }
+    }
+  }
+}
+
+bool launch_copy_variant(hipStream_t s, const void* in, void* out, size_t bytes, int variant, int blocks) {
+  const long long n4 = (long long)(bytes / 16);
+  const kw_f4* i4 = (const kw_f4*)in;
+  kw_f4* o4 = (kw_f4*)out;
+  dim3 g(blocks > 0 ? blocks : 2048), b(256);
+  switch (variant) {
+    case 0: hipLaunchKernelGGL(k_copy, g, b, 0, s, (const float4*)in, (float4*)out, n4); return true;
+    case 1: hipLaunchKernelGGL((k_copy_u<4, false>), g, b, 0, s, i4, o4, n4); return true;
+    case 2: hipLaunchKernelGGL((k_copy_u<8, false>), g, b, 0, s, i4, o4, n4); return true;
+    case 3: hipLaunchKernelGGL((k_copy_u<4, true>), g, b, 0, s, i4, o4, n4); return true;
+    case 4: hipLaunchKernelGGL((k_copy_u<8, true>), g, b, 0, s, i4, o4, n4); return true;
+    default: return false;
+  }
+}
+
+// zero fill (used instead of hipMemsetAsync)
+__global__ void k_zero(kw_f4* __restrict__ p4, long long n4, unsigned char* __restrict__ tail, int ntail) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n4) p4[i] = kw_f4{0.f, 0.f, 0.f, 0.f};
+  if (i < ntail) tail[i] = 0;
+}
+
+void launch_zero(hipStream_t s, void* p, size_t bytes) {
+  if (!bytes) return;
+  // device allocations here are 256-byte aligned; the tail covers sizes not a multiple of 16
+  const long long n4 = (long long)(bytes / 16);
+  const int ntail = (int)(bytes - (size_t)n4 * 16);
+  const long long n = std::max<long long>(n4, ntail);
+  hipLaunchKernelGGL(k_zero, dim3(nblk(n, 256)), dim3(256), 0, s, (kw_f4*)p, n4,
+                     (unsigned char*)p + (size_t)n4 * 16, ntail);
 }
 
 void launch_finish_rms(hipStream_t s, long long n, const double* sum, double count, float* out) {

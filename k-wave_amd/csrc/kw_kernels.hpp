@@ -215,5 +215,9 @@ void launch_absorb_coeffs(hipStream_t s, long long n, const float* c0, float c0s
 void launch_reciprocal(hipStream_t s, long long n, const float* in, float* out);
 void launch_finish_rms(hipStream_t s, long long n, const double* sum, double count, float* out);
 void launch_copy(hipStream_t s, const void* in, void* out, size_t bytes);
+// variant 0: launch_copy's kernel; 1/2: 4/8 loads in flight per thread; 3/4: the same
+// with non-temporal loads and stores.  blocks: grid size (256-thread workgroups)
+bool launch_copy_variant(hipStream_t s, const void* in, void* out, size_t bytes, int variant, int blocks);
+void launch_zero(hipStream_t s, void* p, size_t bytes);
 
 }  // namespace kw

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <utility>
@@ -60,6 +61,14 @@ struct HipError {
 struct OomError {
   size_t bytes;
 };
+struct ArenaFull {
+  size_t need, cap;
+};
+
+constexpr size_t kAllocAlign = 256;
+constexpr size_t alloc_round(size_t bytes) {
+  return ((bytes > 16 ? bytes : 16) + kAllocAlign - 1) / kAllocAlign * kAllocAlign;
+}
 
 // Factor n into radices {8,4,2,3,5,7}; false if another prime divides n.
 bool make_axis(int n, FftAxis* ax) {
@@ -134,7 +143,12 @@ struct kw_sim {
   // medium / operators
   Medium m{};
   SpecTables T{};
-  std::vector<void*> owned;
+  // device allocations: stream-ordered pool allocations, or (arena_base set)
+  // bump allocations inside a caller-provided region that is never freed here
+  std::vector<std::pair<void*, size_t>> owned;
+  size_t bytes_now = 0, bytes_peak = 0;
+  char* arena_base = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
   SrcData psrc, usrc[3];
   uint64_t u_flags[3] = {};
   // sensor
@@ -153,35 +167,70 @@ struct kw_sim {
   float2* ns_xmult = nullptr;
   double* u_rms[3] = {};
   long long nrec_total = 0;  // Nt - start + 1
-  // profiling
+  bool own_stream = true;  // false: the caller's stream (kw_sim_run_noalloc)
+  // profiling: an event pair per launch, resolved when the totals are read
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  struct EvPair {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<EvPair> prof_ev;
+  size_t prof_used = 0;
   double prof_ms[KK_COUNT] = {};
   uint64_t prof_n[KK_COUNT] = {};
 
   template <typename T_>
   T_* alloc(size_t count) {
     void* p = nullptr;
-    size_t bytes = std::max<size_t>(count * sizeof(T_), 16);
-    // stream-ordered: no device-wide synchronisation, so simulations created
-    // or destroyed while others run do not stall them
-    hipError_t e = hipMallocAsync(&p, bytes, stream);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw OomError{bytes};
+    // every allocation is counted rounded up to kAllocAlign (the footprint
+    // formula uses the same rounding)
+    const size_t bytes = alloc_round(count * sizeof(T_));
+    if (arena_base) {
+      if (arena_used + bytes > arena_cap) throw ArenaFull{arena_used + bytes, arena_cap};
+      p = arena_base + arena_used;
+      arena_used += bytes;
+    } else {
+      // stream-ordered: no device-wide synchronisation, so simulations created
+      // or destroyed while others run do not stall them
+      hipError_t e = hipMallocAsync(&p, bytes, stream);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw OomError{bytes};
+      }
     }
-    owned.push_back(p);
+    owned.push_back({p, bytes});
+    bytes_now += bytes;
+    bytes_peak = std::max(bytes_peak, bytes_now);
     return static_cast<T_*>(p);
+  }
+  // Frees a set-up buffer once the kernels that read it are queued (pool
+  // allocations only; an arena keeps it until the end of the call).
+  void release(const void* p) {
+    if (arena_base || !p) return;
+    for (size_t i = 0; i < owned.size(); ++i)
+      if (owned[i].first == p) {
+        HIPCHK(hipFreeAsync(owned[i].first, stream));
+        bytes_now -= owned[i].second;
+        owned.erase(owned.begin() + (long)i);
+        return;
+      }
   }
   template <typename T_>
   T_* upload(const T_* host, size_t count) {
     T_* d = alloc<T_>(count);
     // host buffers may be temporaries of the caller: wait for this stream only
-    HIPCHK(hipMemcpyAsync(d, host, count * sizeof(T_), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    if (count) {
+      HIPCHK(hipMemcpyAsync(d, host, count * sizeof(T_), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    } else {
+      zero(d, sizeof(T_));
+    }
     return d;
   }
-  void zero(void* p, size_t bytes) { HIPCHK(hipMemsetAsync(p, 0, bytes, stream)); }
+  // a fill kernel rather than hipMemsetAsync (the same code path serves
+  // callers that may capture the stream into a graph)
+  void zero(void* p, size_t bytes) { launch_zero(stream, p, bytes); }
 
   long long nspec() const { return (long long)g.Nxh * g.Ny * g.Nz; }
 };
@@ -199,12 +248,19 @@ void free_all(kw_sim* s) {
       (void)hipEventDestroy(e);
     }
   }
-  for (void* p : s->owned) (void)hipFreeAsync(p, s->stream ? s->stream : st);
+  if (!s->arena_base)
+    for (const auto& p : s->owned) (void)hipFreeAsync(p.first, s->stream ? s->stream : st);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   s->owned.clear();
+  s->bytes_now = 0;
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  for (const auto& e : s->prof_ev) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  s->prof_ev.clear();
+  if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
 }
 
 // Launch geometry for the x kernels: P row pairs per workgroup with
@@ -292,22 +348,39 @@ kw_status check_array(const kw_real_array& a, long long N, const char* name, boo
 }
 
 // ---- per-kernel timing ------------------------------------------------------
+// Adds the durations of the recorded launches to the per-kind totals.
+void prof_resolve(kw_sim* s) {
+  if (!s->prof_used) return;
+  HIPCHK(hipEventSynchronize(s->prof_ev[s->prof_used - 1].b));
+  for (size_t i = 0; i < s->prof_used; ++i) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->prof_ev[i].a, s->prof_ev[i].b));
+    s->prof_ms[s->prof_ev[i].kind] += ms;
+    s->prof_n[s->prof_ev[i].kind] += 1;
+  }
+  s->prof_used = 0;
+}
+
+// Brackets one launch with a pair of events on its stream (no host wait).
 struct Prof {
   kw_sim* s;
   hipStream_t st;
-  int kind;
-  Prof(kw_sim* s_, hipStream_t st_, int k) : s(s_), st(st_), kind(k) {
-    if (s->profiling) HIPCHK(hipEventRecord(s->ev0, st));
+  size_t i = 0;
+  Prof(kw_sim* s_, hipStream_t st_, int k) : s(s_), st(st_) {
+    if (!s->profiling) return;
+    if (s->prof_used == 4096) prof_resolve(s);  // bounded event pool
+    if (s->prof_used == s->prof_ev.size()) {
+      kw_sim::EvPair e{nullptr, nullptr, 0};
+      HIPCHK(hipEventCreate(&e.a));
+      HIPCHK(hipEventCreate(&e.b));
+      s->prof_ev.push_back(e);
+    }
+    i = s->prof_used++;
+    s->prof_ev[i].kind = k;
+    HIPCHK(hipEventRecord(s->prof_ev[i].a, st));
   }
   ~Prof() noexcept(false) {
-    if (s->profiling) {
-      HIPCHK(hipEventRecord(s->ev1, st));
-      HIPCHK(hipEventSynchronize(s->ev1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-      s->prof_ms[kind] += ms;
-      s->prof_n[kind] += 1;
-    }
+    if (s->profiling) HIPCHK(hipEventRecord(s->prof_ev[i].b, st));
   }
 };
 
@@ -579,6 +652,7 @@ kw_status setup_source(kw_sim* s, SrcData& d, uint64_t flag, uint64_t mode, uint
       source_spectrum(s, s->stream, u, 1, tmp, comp);
       d.pattern = s->alloc<float>((size_t)s->g.N);
       launch_x_c2r(s->stream, s->g, s->axX, s->twX, tmp, d.pattern, s->xl);
+      s->release(tmp);
     }
   }
   return KW_OK;
@@ -661,8 +735,16 @@ struct TuneCfg {
   int row, pencil_y, pencil_last, plane;
   double ms;
 };
+// One entry per key; the first caller tunes while holding the entry's lock, so
+// concurrent callers of the same shape wait for that result instead of tuning
+// (and overwriting the cache) themselves.
+struct TuneEntry {
+  std::mutex mu;
+  bool done = false;
+  TuneCfg cfg{};
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, TuneCfg> g_tune_cache;
+std::map<TuneKey, std::shared_ptr<TuneEntry>> g_tune_cache;
 
 TuneKey tune_key(const kw_sim* s) {
   return TuneKey{s->g.Nx,        s->g.Ny, s->g.Nz, s->g.kxm, s->m.absorbing, s->m.nonlinear,
@@ -678,54 +760,180 @@ void apply_cfg(kw_sim* s, const TuneCfg& c) {
 
 // back to step 0: the fields carried between steps and every output cleared
 void reset_state(kw_sim* s) {
-  for (const StateBuf& b : state_buffers(s)) HIPCHK(hipMemsetAsync(b.p, 0, b.bytes, s->stream));
+  for (const StateBuf& b : state_buffers(s)) s->zero(b.p, b.bytes);
   HIPCHK(hipStreamSynchronize(s->stream));
   s->t = 0;
 }
 }  // namespace
+
+
+// Device bytes kw_sim_create allocates for a description, following its
+// allocation sequence (same order, same rounding, same early releases):
+// *peak with stream-ordered pool allocations (set-up buffers released),
+// *total when nothing is released (an arena).  False on an invalid grid.
+static bool footprint_plan(const kw_sim_desc* d, uint64_t* peak, uint64_t* total) {
+  *peak = *total = 0;
+  if (!d || d->Nx == 0 || d->Ny == 0 || d->Nz == 0) return false;
+  uint64_t now = 0;
+  auto A = [&](uint64_t bytes) {
+    const uint64_t b = alloc_round((size_t)bytes);
+    now += b;
+    *total += b;
+    *peak = std::max(*peak, now);
+  };
+  auto R = [&](uint64_t bytes) { now -= alloc_round((size_t)bytes); };
+  const uint64_t N = d->Nx * d->Ny * d->Nz;
+  const int dim = d->Nz > 1 ? 3 : 2;
+  const uint64_t Nxh = d->Nx / 2 + 1;
+  const uint64_t nspec = Nxh * d->Ny * d->Nz;
+  const uint64_t rows = d->Ny * d->Nz;
+  const bool want_ns = (d->record_flags & KW_REC_U_NON_STAGGERED) != 0;
+  const bool additive = (d->p_source_flag && d->p_source_mode == 2) ||
+                        ((d->ux_source_flag || d->uy_source_flag || d->uz_source_flag) && d->u_source_mode == 2);
+  const uint64_t nfull[3] = {d->Nx, d->Ny, d->Nz};
+  const uint64_t nout[3] = {Nxh, d->Ny, d->Nz};
+  auto field = [&](const kw_real_array& a) { return a.count > 1; };
+  // twiddles, derivative / shift vectors, u_non_staggered scratch
+  A(d->Nx * 8);
+  A(d->Ny * 8);
+  if (dim == 3) A(d->Nz * 8);
+  for (int a = 0; a < dim; ++a) {
+    A(nout[a] * 8);
+    A(nout[a] * 8);
+    if (want_ns) A(nout[a] * 8);
+  }
+  if (want_ns) {
+    A(Nxh * 8);
+    A(nspec * 8);
+    A(N * 4);
+  }
+  // spectral tables, PML
+  A(nspec * 4);
+  if (additive) A(nspec * 4);
+  if (d->absorbing_flag == 1) {
+    A(nspec * 4);
+    A(nspec * 4);
+  }
+  for (int a = 0; a < dim; ++a) {
+    A(nfull[a] * 4);
+    A(nfull[a] * 4);
+  }
+  if (d->axisymmetric_flag) {
+    const uint64_t L = 2 * d->Ny;
+    A(2 * L * 8);
+    A(L * 8);
+    A(Nxh * L * 4);
+    if (additive) A(Nxh * L * 4);
+    for (int k = 0; k < 3; ++k) A(L * 8);
+    A(d->Ny * 4);
+  }
+  // medium
+  if (field(d->c0)) A(N * 4);
+  if (field(d->rho0)) A(N * 4);
+  const kw_real_array* sg[3] = {&d->rho0_sgx, &d->rho0_sgy, &d->rho0_sgz};
+  for (int a = 0; a < dim; ++a)
+    if (field(*sg[a])) {
+      A(N * 4);
+      A(N * 4);
+      R(N * 4);
+    }
+  if (d->nonlinear_flag && field(d->BonA)) A(N * 4);
+  if (d->absorbing_flag) {
+    const bool fld = field(d->c0) || field(d->alpha_coeff);
+    if (field(d->alpha_coeff)) A(N * 4);
+    A((fld ? N : 1) * 4);
+    A((fld ? N : 1) * 4);
+    if (field(d->alpha_coeff)) R(N * 4);
+  }
+  // state
+  for (int a = 0; a < dim; ++a) {
+    A(N * 4);
+    A(N * 4);
+    A(nspec * 8);
+  }
+  A(nspec * 8);
+  if (d->absorbing_flag == 1) A(N * 4);
+  if (d->p0_source_input) A(N * 4);
+  // sensor and outputs
+  const uint64_t ns = d->sensor_npoints, nsz = std::max<uint64_t>(ns, 1);
+  const uint64_t start = d->record_start_index ? d->record_start_index : 1;
+  const uint64_t nrec = d->Nt >= start ? d->Nt - start + 1 : 0;
+  const uint64_t rf = d->record_flags;
+  A((rows + 1) * 4);
+  A(nsz * 8);
+  if (rf & KW_REC_P) A(nsz * nrec * 4);
+  if (rf & KW_REC_P_MAX) A(nsz * 4);
+  if (rf & KW_REC_P_MIN) A(nsz * 4);
+  if (rf & KW_REC_P_RMS) A(2 * nsz * 8);
+  if (rf & KW_REC_P_MAX_ALL) A(N * 4);
+  if (rf & KW_REC_P_MIN_ALL) A(N * 4);
+  if (rf & KW_REC_P_FINAL) A(N * 4);
+  for (int a = 0; a < dim; ++a) {
+    if (rf & KW_REC_U) A(nsz * nrec * 4);
+    if (rf & KW_REC_U_MAX) A(nsz * 4);
+    if (rf & KW_REC_U_MIN) A(nsz * 4);
+    if (rf & KW_REC_U_RMS) A(2 * nsz * 8);
+    if (rf & KW_REC_U_MAX_ALL) A(N * 4);
+    if (rf & KW_REC_U_MIN_ALL) A(N * 4);
+    if (rf & KW_REC_U_NON_STAGGERED) A(nsz * nrec * 4);
+  }
+  // sources (setup_source)
+  auto src = [&](uint64_t flag, uint64_t mode, uint64_t many, uint64_t np) {
+    A(flag * (many ? np : 1) * 4);
+    if (mode == 2) {
+      if (many) {
+        A(nspec * 8);
+      } else {
+        A(nspec * 8);
+        A(N * 4);
+        R(nspec * 8);
+      }
+    }
+  };
+  if (d->p_source_flag) {
+    A((rows + 1) * 4);
+    A(d->p_source_npoints * 8);
+    src(d->p_source_flag, d->p_source_mode, d->p_source_many, d->p_source_npoints);
+  }
+  if (d->transducer_source_flag) {
+    A((rows + 1) * 4);
+    A(d->u_source_npoints * 8);
+    A(d->transducer_source_length * 4);
+    A(d->u_source_npoints * 4);
+  }
+  const uint64_t uf[3] = {d->ux_source_flag, d->uy_source_flag, dim == 3 ? d->uz_source_flag : 0};
+  if (uf[0] || uf[1] || uf[2]) {
+    A((rows + 1) * 4);
+    A(d->u_source_npoints * 8);
+    bool shared = false;
+    for (int a = 0; a < dim; ++a) {
+      if (!uf[a]) continue;
+      if (shared && d->u_source_mode == 2 && !d->u_source_many && !d->axisymmetric_flag) {
+        A(uf[a] * 4);
+        continue;
+      }
+      src(uf[a], d->u_source_mode, d->u_source_many, d->u_source_npoints);
+      shared = d->u_source_mode == 2 && !d->u_source_many;
+    }
+  }
+  return true;
+}
 
 extern "C" {
 
 const char* kw_last_error(void) { return g_err.c_str(); }
 
 uint64_t kw_sim_footprint_bytes(const kw_sim_desc* d) {
-  if (!d || d->Nx == 0 || d->Ny == 0 || d->Nz == 0) return 0;
-  const uint64_t N = d->Nx * d->Ny * d->Nz;
-  const uint64_t dim = d->Nz > 1 ? 3 : 2;
-  const uint64_t nspec = (d->Nx / 2 + 1) * d->Ny * d->Nz;
-  uint64_t b = 0;
-  b += (2 * dim + 1) * N * 4;                  // u, rho, srho
-  b += (1 + dim) * nspec * 8;                  // Pk, G
-  b += 4 * nspec * 4;                          // spectral tables
-  auto fld = [&](const kw_real_array& a) { return a.count > 1 ? N * 4 : 0; };
-  b += fld(d->c0) + fld(d->rho0) + fld(d->rho0_sgx) + fld(d->rho0_sgy) + (dim == 3 ? fld(d->rho0_sgz) : 0);
-  b += fld(d->BonA) + 2 * fld(d->alpha_coeff) + (d->p0_source_input ? N * 4 : 0);
-  const uint64_t start = d->record_start_index ? d->record_start_index : 1;
-  const uint64_t nrec = d->Nt >= start ? d->Nt - start + 1 : 0;
-  const uint64_t ns = d->sensor_npoints;
-  if (d->record_flags & KW_REC_P) b += ns * nrec * 4;
-  if (d->record_flags & KW_REC_U) b += dim * ns * nrec * 4;
-  if (d->record_flags & KW_REC_U_NON_STAGGERED) b += dim * ns * nrec * 4 + nspec * 8 + N * 4;
-  if (d->record_flags & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL | KW_REC_P_FINAL)) b += 3 * N * 4;
-  if (d->record_flags & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL)) b += 2 * dim * N * 4;
-  b += 8 * ns * 16;
-  // sources: signals, and the corrected pattern / spectrum of additive ones
-  auto src = [&](uint64_t flag, uint64_t mode, uint64_t many, uint64_t np) {
-    if (!flag) return (uint64_t)0;
-    uint64_t c = flag * (many ? np : 1) * 4;
-    if (mode == 2) c += nspec * 8 + (many ? 0 : N * 4);
-    return c;
-  };
-  b += src(d->p_source_flag, d->p_source_mode, d->p_source_many, d->p_source_npoints);
-  b += src(d->ux_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
-  b += src(d->uy_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
-  if (dim == 3) b += src(d->uz_source_flag, d->u_source_mode, d->u_source_many, d->u_source_npoints);
-  if (d->transducer_source_flag) b += d->transducer_source_length * 4 + d->u_source_npoints * 16;
-  if (d->axisymmetric_flag) b += (d->Nx / 2 + 1) * 2 * d->Ny * 8 + d->Ny * 64;  // radial tables
-  return b;
+  uint64_t peak = 0, total = 0;
+  footprint_plan(d, &peak, &total);
+  return peak;
 }
 
-kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
+// kw_sim_create with every device buffer either allocated from the stream-ordered
+// pool (arena == nullptr) or placed in [arena, arena + arena_cap); ext_stream:
+// run on the caller's stream instead of creating one.
+static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap, bool use_ext,
+                             hipStream_t ext_stream, kw_sim** out) {
   if (!d || !out) return fail(KW_ERR_INVALID, "null argument");
   *out = nullptr;
   if (d->abi_version != KW_ABI_VERSION)
@@ -875,9 +1083,14 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
       // (measured fastest there; otherwise 256 for Nx >= 256 and 128 below)
       if (g.kxm && !std::getenv("KW_ROW_NT")) s->xl.nt = 256;
     }
-    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&s->ev0));
-    HIPCHK(hipEventCreate(&s->ev1));
+    if (use_ext) {
+      s->stream = ext_stream;
+      s->own_stream = false;
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    }
+    s->arena_base = arena;
+    s->arena_cap = arena ? arena_cap : 0;
     hipStream_t st0 = s->stream;
 
     // twiddle tables (FP64 -> FP32)
@@ -989,6 +1202,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
         float* tmp = s->upload(sg[a]->data, (size_t)N);
         float* inv = s->alloc<float>((size_t)N);
         launch_reciprocal(st0, N, tmp, inv);
+        s->release(tmp);
         m.rsg[a].f = inv;
       }
     }
@@ -1002,6 +1216,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
       float* eta = s->alloc<float>((size_t)n);
       launch_absorb_coeffs(st0, n, field ? c0d : nullptr, d->c0.data[0], ad, d->alpha_coeff.data[0],
                            d->alpha_power, (int)d->absorbing_flag, tau, eta);
+      s->release(ad);
       if (field) {
         m.tau.f = tau;
         m.eta.f = eta;
@@ -1036,7 +1251,7 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     std::vector<int> srows;
     build_rows(slin, g, srows);
     s->s_row_ptr = s->upload(srows.data(), srows.size());
-    s->s_lin = s->upload(slin.data(), std::max<size_t>(slin.size(), 1));
+    s->s_lin = s->upload(slin.data(), slin.size());  // (no sensor points: one zeroed entry)
     s->nrec_total = s->Nt - s->start + 1;
     const size_t nsz = std::max(s->ns, 1);
     const uint64_t rf = s->rec;
@@ -1132,8 +1347,24 @@ kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
     free_all(s);
     delete s;
     return fail(KW_ERR_OOM, "device allocation of %zu bytes failed", e.bytes);
+  } catch (const ArenaFull& e) {
+    free_all(s);
+    delete s;
+    return fail(KW_ERR_UNSUPPORTED, "simulation needs more than the %zu-byte scratch region (%zu bytes)", e.cap,
+                e.need);
   }
   *out = s;
+  return KW_OK;
+}
+
+kw_status kw_sim_create(const kw_sim_desc* d, kw_sim** out) {
+  return create_impl(d, nullptr, 0, false, nullptr, out);
+}
+
+kw_status kw_sim_memory(const kw_sim* s, uint64_t* allocated, uint64_t* peak) {
+  if (!s) return fail(KW_ERR_INVALID, "null handle");
+  if (allocated) *allocated = s->bytes_now;
+  if (peak) *peak = s->bytes_peak;
   return KW_OK;
 }
 
@@ -1298,16 +1529,18 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
     HIPCHK(hipSetDevice(s->device));
     const TuneKey key = tune_key(s);
     TuneCfg best{s->xl.nt, s->tly.nt, s->tll.nt, s->plane_nt, 1e30};
-    bool cached = false;
+    std::shared_ptr<TuneEntry> ent;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      auto it = g_tune_cache.find(key);
-      if (it != g_tune_cache.end()) {
-        best = it->second;
-        cached = true;
-      }
+      std::shared_ptr<TuneEntry>& slot = g_tune_cache[key];
+      if (!slot) slot = std::make_shared<TuneEntry>();
+      ent = slot;
     }
-    if (!cached) {
+    std::lock_guard<std::mutex> klk(ent->mu);
+    if (ent->done) best = ent->cfg;
+    if (!ent->done) {
+      if (!s->ev0) HIPCHK(hipEventCreate(&s->ev0));
+      if (!s->ev1) HIPCHK(hipEventCreate(&s->ev1));
       std::vector<int> rows{0};
       if (s->xl.nt) rows = s->g.Nx <= 96 ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256};
       // (y pass, last axis) workgroup sizes
@@ -1335,8 +1568,8 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
             c.ms = ms / std::max<long long>(1, steps - 1);
             if (c.ms < best.ms) best = c;
           }
-      std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_tune_cache[key] = best;
+      ent->cfg = best;
+      ent->done = true;
     }
     apply_cfg(s, best);
     reset_state(s);
@@ -1352,6 +1585,7 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
 kw_status kw_sim_set_profiling(kw_sim* s, int enable) {
   if (!s) return fail(KW_ERR_INVALID, "null handle");
   s->profiling = enable != 0;
+  s->prof_used = 0;  // launches recorded but not yet read are dropped
   for (int k = 0; k < KK_COUNT; ++k) {
     s->prof_ms[k] = 0.0;
     s->prof_n[k] = 0;
@@ -1361,6 +1595,13 @@ kw_status kw_sim_set_profiling(kw_sim* s, int enable) {
 
 int kw_sim_profile_get(kw_sim* s, const char** names, double* total_ms, uint64_t* launches, int cap) {
   if (!s) return 0;
+  try {
+    HIPCHK(hipSetDevice(s->device));
+    prof_resolve(s);
+  } catch (const HipError& e) {
+    fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+    return -1;
+  }
   for (int k = 0; k < KK_COUNT && k < cap; ++k) {
     if (names) names[k] = kKindNames[k];
     if (total_ms) total_ms[k] = s->prof_ms[k];
@@ -1369,35 +1610,56 @@ int kw_sim_profile_get(kw_sim* s, const char** names, double* total_ms, uint64_t
   return KK_COUNT;
 }
 
-kw_status kw_diag_copy(uint64_t bytes, int device, double* ms) {
+kw_status kw_diag_copy(uint64_t bytes, int device, double* ms) { return kw_diag_copy_ex(bytes, device, 0, 0, ms); }
+
+kw_status kw_diag_copy_ex(uint64_t bytes, int device, int variant, int blocks, double* ms) {
+  if (variant < 0 || variant > 4 || blocks < 0 || bytes < 16)
+    return fail(KW_ERR_INVALID, "copy variant %d, %d blocks, %llu bytes", variant, blocks, (unsigned long long)bytes);
   void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  kw_status st = KW_OK;
   try {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipMalloc(&a, bytes));
     HIPCHK(hipMalloc(&b, bytes));
     HIPCHK(hipMemset(a, 0, bytes));
-    hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    launch_copy(nullptr, a, b, (size_t)bytes);  // first launch: code-object load, cold caches
+    launch_copy_variant(nullptr, a, b, (size_t)bytes, variant, blocks);  // code-object load, cold caches
     HIPCHK(hipEventRecord(e0, nullptr));
     const int reps = 10;
-    for (int r = 0; r < reps; ++r) launch_copy(nullptr, (r & 1) ? b : a, (r & 1) ? a : b, (size_t)bytes);
+    for (int r = 0; r < reps; ++r)
+      launch_copy_variant(nullptr, (r & 1) ? b : a, (r & 1) ? a : b, (size_t)bytes, variant, blocks);
     HIPCHK(hipEventRecord(e1, nullptr));
     HIPCHK(hipEventSynchronize(e1));
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, e0, e1));
     if (ms) *ms = t / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   } catch (const HipError& e) {
-    if (a) (void)hipFree(a);
-    if (b) (void)hipFree(b);
-    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+    st = fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
   }
-  (void)hipFree(a);
-  (void)hipFree(b);
-  return KW_OK;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return st;
 }
 
 }  // extern "C"
+
+namespace kw {
+// For kw_noalloc.hip: a simulation whose every device buffer lies in
+// [arena, arena + cap), run on `stream` (which may be the null stream).
+kw_status sim_create_in(const kw_sim_desc* d, void* arena, size_t cap, hipStream_t stream, kw_sim** out) {
+  return create_impl(d, static_cast<char*>(arena), cap, true, stream, out);
+}
+// Bytes such a simulation occupies in the arena (nothing is released there); 0 if invalid.
+uint64_t sim_arena_bytes(const kw_sim_desc* d) {
+  uint64_t peak = 0, total = 0;
+  return footprint_plan(d, &peak, &total) ? total : 0;
+}
+kw_status set_error(kw_status st, const char* msg) {
+  g_err = msg ? msg : "";
+  return st;
+}
+}  // namespace kw

@@ -39,7 +39,7 @@ def _time_series(name: str) -> bool:
 EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
     "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
-    "kw_sim_footprint_bytes", "kw_diag_copy", "kw_sim_stream",
+    "kw_sim_footprint_bytes", "kw_diag_copy", "kw_diag_copy_ex", "kw_sim_stream", "kw_sim_memory",
     "kw_sim_checkpoint_bytes", "kw_sim_checkpoint_save", "kw_sim_checkpoint_load", "kw_sim_autotune",
     "kw_server_create", "kw_server_submit", "kw_server_poll", "kw_server_wait", "kw_server_request_info",
     "kw_server_request_error", "kw_server_output_bytes", "kw_server_fetch", "kw_server_release",
@@ -142,6 +142,10 @@ def load_library(path: str = LIB_PATH):
     lib.kw_sim_autotune.restype = C.c_int
     lib.kw_sim_stream.argtypes = [C.c_void_p]
     lib.kw_sim_stream.restype = C.c_void_p
+    lib.kw_diag_copy_ex.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    lib.kw_diag_copy_ex.restype = C.c_int
+    lib.kw_sim_memory.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.kw_sim_memory.restype = C.c_int
     vp, u64 = C.c_void_p, C.c_uint64
     for name, args, res in (
             ("kw_server_create", [C.POINTER(KwServerConfig), C.POINTER(vp)], C.c_int),
@@ -353,6 +357,12 @@ class Simulation:
                                                   C.byref(pl)))
         return {"row_nt": r.value, "pencil_nt": pe.value, "plane_nt": pl.value}
 
+    def memory(self) -> dict:
+        """Device bytes held now and at most since creation (kw_sim_memory)."""
+        a, pk = C.c_uint64(), C.c_uint64()
+        _check(self.lib, self.lib.kw_sim_memory(self.handle, C.byref(a), C.byref(pk)))
+        return {"allocated": a.value, "peak": pk.value}
+
     def set_profiling(self, on: bool = True):
         _check(self.lib, self.lib.kw_sim_set_profiling(self.handle, 1 if on else 0))
 
@@ -397,11 +407,12 @@ def footprint_bytes(f: dict, record=("p",), record_start_index: int = 1) -> int:
     return n
 
 
-def diag_copy(nbytes: int, device: int = 0) -> float:
-    """Time one plain device copy of nbytes (counter calibration); returns ms."""
+def diag_copy(nbytes: int, device: int = 0, variant: int = 0, blocks: int = 0) -> float:
+    """Time one device copy of nbytes (kw_diag_copy_ex; variant 0 is the
+    counter-calibration kernel); returns ms per copy."""
     lib = load_library()
     ms = C.c_double()
-    _check(lib, lib.kw_diag_copy(int(nbytes), device, C.byref(ms)))
+    _check(lib, lib.kw_diag_copy_ex(int(nbytes), device, int(variant), int(blocks), C.byref(ms)))
     return ms.value
 
 

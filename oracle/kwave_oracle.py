@@ -40,13 +40,66 @@ stability bound.  Functions without such a check say so in their docstring.
 
 from __future__ import annotations
 
+import concurrent.futures
 import math
+import os
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
 from scipy import fft as sfft
 
 WORKERS = -1  # scipy.fft workers: all cores
+
+# Elementwise statements on large grids are evaluated slab by slab (along the
+# first, z, axis) on all cores; numpy releases the GIL inside its loops.  The
+# per-element arithmetic is the statement's own, so the values are those of
+# the whole-array evaluation.
+PAR_MIN_ELEMS = 1 << 21
+_pool = None
+
+
+def _threads():
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def pev(fn, *xs):
+    """fn(*xs) for an elementwise fn, evaluated over z-slabs in parallel.
+
+    Operands are scalars or arrays broadcastable to the 3-D result; arrays
+    spanning the result's first axis are split along it, the others (extent
+    1 there, or fewer dimensions) are passed whole to every slab."""
+    shapes = [np.shape(x) for x in xs]
+    full = np.broadcast_shapes(*shapes)
+    if len(full) != 3 or full[0] < 2 or int(np.prod(full)) < PAR_MIN_ELEMS or _threads() == 1:
+        return fn(*xs)
+    global _pool
+    if _pool is None:
+        _pool = concurrent.futures.ThreadPoolExecutor(_threads())
+    n0 = full[0]
+    nch = min(n0, 4 * _threads())
+    bounds = [n0 * i // nch for i in range(nch + 1)]
+
+    def args(i):
+        a, b = bounds[i], bounds[i + 1]
+        return [x[a:b] if (isinstance(x, np.ndarray) and x.ndim == 3 and x.shape[0] == n0) else x for x in xs]
+
+    first = np.asarray(fn(*args(0)))
+    out = np.empty(full, dtype=first.dtype)
+    out[bounds[0]:bounds[1]] = first
+
+    def part(i):
+        out[bounds[i]:bounds[i + 1]] = fn(*args(i))
+
+    list(_pool.map(part, range(1, nch)))
+    return out
+
+
+def psum(arrs):
+    """sum(arrs) of same-shape arrays, slab-parallel (left to right, as sum())."""
+    if len(arrs) == 1:
+        return arrs[0]
+    return pev(lambda *a: sum(a), *arrs)
 
 RECORD_NAMES = (
     "p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
@@ -126,10 +179,10 @@ def _int(v) -> int:
 
 
 def _field_or_scalar(v, shape):
-    a = np.asarray(v, dtype=np.float64)
+    a = np.asarray(v)
     if a.size == 1:
         return float(a.reshape(-1)[0])
-    return a.reshape(shape)
+    return pev(lambda x: np.asarray(x, dtype=np.float64), a.reshape(shape))
 
 
 @dataclass
@@ -269,13 +322,13 @@ class Operators:
         kx = kv[0][None, None, :]
         ky = kv[1][None, :, None]
         kz = kv[2][:, None, None]
-        self.k = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+        self.k = pev(lambda kx, ky, kz: np.sqrt(kx ** 2 + ky ** 2 + kz ** 2), kx, ky, kz)
         shp = [kx, ky, kz]
         # ddx_k_shift_pos/neg (3D.m:627-634), broadcast along their axis
         self.d_pos = [1j * shp[a] * np.exp(1j * shp[a] * d[a] / 2) for a in range(pb.dim)]
         self.d_neg = [1j * shp[a] * np.exp(-1j * shp[a] * d[a] / 2) for a in range(pb.dim)]
-        self.kappa = sinc(pb.c_ref * self.k * pb.dt / 2)                 # 3D.m:652
-        self.source_kappa = np.cos(pb.c_ref * self.k * pb.dt / 2)       # 3D.m:654
+        self.kappa = pev(lambda k: sinc(pb.c_ref * k * pb.dt / 2), self.k)              # 3D.m:652
+        self.source_kappa = pev(lambda k: np.cos(pb.c_ref * k * pb.dt / 2), self.k)    # 3D.m:654
         # PML (3D.m:617-622)
         self.pml = []
         self.pml_sg = []
@@ -290,18 +343,20 @@ class Operators:
         self.absorb_nabla2 = None
         if pb.absorbing_flag == 1:
             y = pb.alpha_power
-            a_np = db2neper(pb.alpha_coeff, y)
-            with np.errstate(divide="ignore"):
-                n1 = self.k ** (y - 2)
-            n1[np.isinf(n1)] = 0.0
-            self.absorb_nabla1 = n1
-            with np.errstate(divide="ignore"):
-                n2 = self.k ** (y - 1)
-            n2[np.isinf(n2)] = 0.0
-            self.absorb_nabla2 = n2
+            a_np = pev(lambda a: db2neper(a, y), pb.alpha_coeff)
+
+            def kpow(k, e):
+                # k^e with the Inf at k = 0 set to 0 (createAbsorptionVariables.m:33-41)
+                with np.errstate(divide="ignore"):
+                    n = k ** e
+                n[np.isinf(n)] = 0.0
+                return n
+
+            self.absorb_nabla1 = pev(lambda k: kpow(k, y - 2), self.k)
+            self.absorb_nabla2 = pev(lambda k: kpow(k, y - 1), self.k)
             c0 = pb.c0
-            self.absorb_tau = -2.0 * a_np * np.power(c0, y - 1)
-            self.absorb_eta = 2.0 * a_np * np.power(c0, y) * math.tan(math.pi * y / 2)
+            self.absorb_tau = pev(lambda a, c: -2.0 * a * np.power(c, y - 1), a_np, c0)
+            self.absorb_eta = pev(lambda a, c: 2.0 * a * np.power(c, y) * math.tan(math.pi * y / 2), a_np, c0)
         elif pb.absorbing_flag == 2:
             a_np = db2neper(pb.alpha_coeff, 2)
             self.absorb_tau = -2.0 * a_np * pb.c0
@@ -323,16 +378,19 @@ class Spectral:
         self.axes = (-1, -2) if pb.dim == 2 else (-1, -2, -3)
         self.shape = pb.shape
 
+        self.workers = WORKERS
+
     def fwd(self, f):
         if self.full:
-            return sfft.fftn(f, axes=self.axes, workers=WORKERS)
-        return sfft.rfftn(f, axes=self.axes[::-1], workers=WORKERS)
+            return sfft.fftn(f, axes=self.axes, workers=self.workers)
+        return sfft.rfftn(f, axes=self.axes[::-1], workers=self.workers)
 
     def inv_real(self, F):
+        # F is always a temporary of the caller: the transform may overwrite it
         if self.full:
-            return np.real(sfft.ifftn(F, axes=self.axes, workers=WORKERS))
+            return np.real(sfft.ifftn(F, axes=self.axes, workers=self.workers, overwrite_x=True))
         sh = [self.shape[a] for a in self.axes[::-1]]
-        return sfft.irfftn(F, s=sh, axes=self.axes[::-1], workers=WORKERS)
+        return sfft.irfftn(F, s=sh, axes=self.axes[::-1], workers=self.workers, overwrite_x=True)
 
 
 class AxisymOps:
@@ -418,12 +476,14 @@ def nonlinear_density(srho, rho0):
 
 
 def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: bool = False,
-        nsteps: int | None = None) -> dict:
+        nsteps: int | None = None, timings: dict | None = None) -> dict:
     """Run the time loop; return outputs named and shaped like the output file.
 
     kspaceFirstOrder3D.m:767-1206 (2D.m:710-1042 for Nz == 1).
     ``record`` holds names from RECORD_NAMES; ``nsteps`` truncates the loop
-    (the outputs then cover steps 1..nsteps).
+    (the outputs then cover steps 1..nsteps).  ``timings`` (a dict) receives
+    the wall time of every step ("step") and of its source-injection
+    branches ("source"), in seconds.
     """
     pb = decode(inp)
     ops = Operators(pb, full_complex)
@@ -438,7 +498,7 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
             raise ValueError(f"unknown record option {r}")
 
     c0, rho0 = pb.c0, pb.rho0
-    rsg_inv = [1.0 / r for r in pb.rho0_sg]
+    rsg_inv = [pev(lambda r: 1.0 / r, r) for r in pb.rho0_sg]
     pml = [ops.bcast_pml(ops.pml[a], a) for a in range(dim)]
     pml_sg = [ops.bcast_pml(ops.pml_sg[a], a) for a in range(dim)]
 
@@ -485,13 +545,19 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
     p_rms_acc = np.zeros(ns)
     u_rms_acc = [np.zeros(ns) for _ in range(dim)]
 
+    if timings is not None:
+        timings.setdefault("step", [])
+        timings.setdefault("source", [])
     for t_index in range(1, Nt + 1):
+        t_step0 = time.perf_counter()
+        t_src = 0.0
         # velocity update (3D.m:790-801; AS.m:777-784)
         grads = axs.grad_p(p) if axs else None
         for a in range(dim):
-            grad = grads[a] if axs else sp.inv_real(ops.d_pos[a] * ops.kappa * p_k)
-            u[a] = pml_sg[a] * (pml_sg[a] * u[a] - dt * rsg_inv[a] * grad)
+            grad = grads[a] if axs else sp.inv_real(pev(lambda d, k, f: d * k * f, ops.d_pos[a], ops.kappa, p_k))
+            u[a] = pev(lambda ps, ua, r, g: ps * (ps * ua - dt * r * g), pml_sg[a], u[a], rsg_inv[a], grad)
         # velocity sources (3D.m:804-884)
+        t0 = time.perf_counter()
         for a in range(dim):
             if pb.u_source_flag[a] >= t_index:
                 vals = source_values(pb.u_source_input[a], u_idx.size, t_index)
@@ -504,26 +570,28 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
                     if axs:
                         sm = axs.source_correction(sm.reshape(shape), "wswa" if a == 0 else "hahs")
                     else:
-                        sm = sp.inv_real(ops.source_kappa * sp.fwd(sm.reshape(shape)))
-                    u[a] = u[a] + sm
+                        sm = sp.inv_real(pev(lambda k, f: k * f, ops.source_kappa, sp.fwd(sm.reshape(shape))))
+                    u[a] = pev(lambda x, y: x + y, u[a], sm)
                 else:
                     uf[u_idx] = uf[u_idx] + vals
         # transducer source (3D.m:889-902): one signal, a per-point delay, x velocity only
         if pb.extras.get("transducer_flag", 0) >= t_index:
             uf = u[0].reshape(-1)
             uf[u_idx] = uf[u_idx] + pb.extras["transducer_signal"][pb.extras["transducer_delay0"] + t_index - 1]
+        t_src += time.perf_counter() - t0
         # divergence (3D.m:905-907)
-        du = axs.div(u[0], u[1]) if axs else [sp.inv_real(ops.d_neg[a] * ops.kappa * sp.fwd(u[a]))
-                                              for a in range(dim)]
+        du = axs.div(u[0], u[1]) if axs else [
+            sp.inv_real(pev(lambda d, k, f: d * k * f, ops.d_neg[a], ops.kappa, sp.fwd(u[a]))) for a in range(dim)]
         # mass conservation (3D.m:910-925)
         if not pb.nonlinear:
             for a in range(dim):
-                rho[a] = pml[a] * (pml[a] * rho[a] - dt * rho0 * du[a])
+                rho[a] = pev(lambda pm, ra, r0, d: pm * (pm * ra - dt * r0 * d), pml[a], rho[a], rho0, du[a])
         else:
-            rpr = nonlinear_density(sum(rho), rho0)
+            rpr = nonlinear_density(psum(rho), rho0)
             for a in range(dim):
-                rho[a] = pml[a] * (pml[a] * rho[a] - dt * rpr * du[a])
+                rho[a] = pev(lambda pm, ra, r0, d: pm * (pm * ra - dt * r0 * d), pml[a], rho[a], rpr, du[a])
         # pressure source (3D.m:928-960)
+        t0 = time.perf_counter()
         if pb.p_source_flag >= t_index:
             vals = source_values(pb.p_source_input, p_idx.size, t_index)
             if pb.p_source_mode == 0:
@@ -535,28 +603,31 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
                 if axs:
                     sm = axs.source_correction(sm.reshape(shape), "wswa")
                 else:
-                    sm = sp.inv_real(ops.source_kappa * sp.fwd(sm.reshape(shape)))
+                    sm = sp.inv_real(pev(lambda k, f: k * f, ops.source_kappa, sp.fwd(sm.reshape(shape))))
                 for a in range(dim):
-                    rho[a] = rho[a] + sm
+                    rho[a] = pev(lambda x, y: x + y, rho[a], sm)
             else:
                 for a in range(dim):
                     rf = rho[a].reshape(-1)
                     rf[p_idx] = rf[p_idx] + vals
+        t_src += time.perf_counter() - t0
         # equation of state (3D.m:963-1017)
-        srho = sum(rho)
+        srho = psum(rho)
         if pb.absorbing_flag == 1:
-            sdu = sum(du)
-            p = c0 ** 2 * (
-                srho
-                + ops.absorb_tau * sp.inv_real(ops.absorb_nabla1 * sp.fwd(rho0 * sdu))
-                - ops.absorb_eta * sp.inv_real(ops.absorb_nabla2 * sp.fwd(srho)))
+            sdu = psum(du)
+            t1 = sp.inv_real(pev(lambda n, f: n * f, ops.absorb_nabla1, sp.fwd(pev(lambda r, d: r * d, rho0, sdu))))
+            t2 = sp.inv_real(pev(lambda n, f: n * f, ops.absorb_nabla2, sp.fwd(srho)))
+            p = pev(lambda c, sr, tau, a, eta, b: c ** 2 * (sr + tau * a - eta * b),
+                    c0, srho, ops.absorb_tau, t1, ops.absorb_eta, t2)
+            del t1, t2
         elif pb.absorbing_flag == 2:
             p = c0 ** 2 * (srho + ops.absorb_tau * rho0 * sum(du))
         else:
             p = c0 ** 2 * srho
         if pb.nonlinear:
-            p = p + c0 ** 2 * (pb.BonA * srho ** 2 / (2.0 * rho0))
-        p = np.broadcast_to(p, shape).copy()
+            p = pev(lambda pp, c, b, sr, r0: pp + c ** 2 * (b * sr ** 2 / (2.0 * r0)), p, c0, pb.BonA, srho, rho0)
+        if np.shape(p) != shape:
+            p = np.broadcast_to(p, shape).copy()
         # initial pressure (3D.m:1021-1035)
         if t_index == 1 and pb.p0 is not None:
             p = pb.p0.copy()
@@ -609,6 +680,9 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
                 if "u_min_all" in rec:
                     k = "u" + nm + "_min_all"
                     out[k] = u[a].copy() if fi == 1 else np.minimum(out[k], u[a])
+        if timings is not None:
+            timings["step"].append(time.perf_counter() - t_step0)
+            timings["source"].append(t_src)
     if "p_final" in rec:
         out["p_final"] = p.copy()
     if "u_final" in rec:
