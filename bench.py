@@ -296,7 +296,7 @@ def cpu_baseline(f: dict, rec, steps: int, nt_full: int) -> dict:
                 "sample": f"skipped: the FP64 oracle needs ~{need / 2**30:.0f} GiB, {avail / 2**30:.0f} GiB available"}
     tm = {}
     t0 = time.perf_counter()
-    kwave_oracle.run(f, record=rec, nsteps=steps, timings=tm)
+    kwave_oracle.run(f, record=rec, nsteps=steps, timings=tm, progress=lambda m: log(f"cpu baseline: {m}"))
     wall = time.perf_counter() - t0
     t_act = float(np.mean(tm["step"]))
     t_src = float(np.mean(tm["source"]))

@@ -195,13 +195,28 @@ kw_status kw_sim_set_profiling(kw_sim* sim, int enable);
 int kw_sim_profile_get(kw_sim* sim, const char** names, double* total_ms, uint64_t* launches, int cap);
 
 /* Launch-geometry autotuner: times each candidate set of workgroup sizes of
- * the row, pencil and plane kernels over `trial_steps` steps of this
+ * the row, pencil and plane kernels and each radix plan of the pencils
+ * (coordinate descent from the defaults) over `trial_steps` steps of this
  * simulation, keeps the fastest (cached per grid shape and physics branch for
- * the rest of the process) and resets the simulation to step 0.  Call before
+ * the rest of the process) and resets the simulation to step 0.  The full
+ * chosen configuration is reported by kw_sim_launch_config.  Call before
  * kw_sim_run.  The chosen sizes are returned (0: kernel kind not used;
  * pencil_nt is the y pass's, or the last axis's when there is no y pass).
  * No counterpart in the reference (its binaries have fixed launch shapes). */
 kw_status kw_sim_autotune(kw_sim* sim, int trial_steps, int* row_nt, int* pencil_nt, int* plane_nt);
+
+/* The launch configuration a handle runs with (after kw_sim_autotune, the
+ * tuned one): workgroup sizes of the x-row kernels, the y pass, the
+ * last-axis pass and the plane pass (0: kind not used by this grid), and the
+ * radix plans of the two pencil passes (0: radices 16, 8, 4, 2, 3, 5, 7
+ * greedily; 1: radix 8 at most).  No counterpart in the reference. */
+typedef struct kw_launch_config {
+  int32_t row_nt, ypass_nt, last_nt, plane_nt;
+  int32_t ypass_plan, last_plan;
+  int32_t kx_major; /* 1: kx-major spectra and the plane pass */
+  int32_t reserved;
+} kw_launch_config;
+kw_status kw_sim_launch_config(const kw_sim* sim, kw_launch_config* config);
 
 /* Device memory (bytes) a description would need at its peak during
  * kw_sim_create and after it; 0 on invalid input.  Follows kw_sim_create's

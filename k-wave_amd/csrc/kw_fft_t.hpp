@@ -17,31 +17,36 @@
 
 namespace kwt {
 
-// 16 is split 4 x 4 when it would be the last stage of a longer transform
-// (256 = 16 x 4 x 4): the last stage's radix sets the registers each thread
-// holds between a C2R and its R2C, and a radix-16 last stage costs 16 of them.
-constexpr int next_radix(int m) {
+// Radix plans.  Plan 0: greedy 16, 8, 4, 2, then 3, 5, 7, with a 16 that
+// would be the last stage of a longer transform split 4 x 4 (256 = 16 x 4 x 4:
+// the last stage's radix sets the registers each thread holds between a C2R
+// and its R2C, and a radix-16 last stage costs 16 of them).  Plan 1: the same
+// with radix 8 at most (more stages, fewer registers per butterfly).
+constexpr int next_radix(int m, int pl = 0) {
+  if (pl == 1)
+    return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5
+         : (m % 7 == 0) ? 7 : 0;
   return (m == 16) ? 4 : (m % 16 == 0) ? 16 : (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2
        : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : (m % 7 == 0) ? 7 : 0;
 }
-constexpr int num_stages(int n) {
+constexpr int num_stages(int n, int pl = 0) {
   int s = 0;
   while (n > 1) {
-    const int r = next_radix(n);
+    const int r = next_radix(n, pl);
     if (r == 0) return -1;
     n /= r;
     ++s;
   }
   return s;
 }
-constexpr int radix_of(int n, int s) {
-  for (int i = 0; i < s; ++i) n /= next_radix(n);
-  return next_radix(n);
+constexpr int radix_of(int n, int s, int pl = 0) {
+  for (int i = 0; i < s; ++i) n /= next_radix(n, pl);
+  return next_radix(n, pl);
 }
-constexpr int ns_of(int n, int s) {
+constexpr int ns_of(int n, int s, int pl = 0) {
   int p = 1;
   for (int i = 0; i < s; ++i) {
-    const int r = next_radix(n);
+    const int r = next_radix(n, pl);
     p *= r;
     n /= r;
   }
@@ -53,16 +58,16 @@ constexpr int ns_of(int n, int s) {
 // the forward plan writes (both have N / R_{S-1} butterflies of stride
 // N / R_{S-1}), so a forward transform's outputs can feed an inverse
 // transform from registers.
-constexpr int radix_rev(int n, int s) { return radix_of(n, num_stages(n) - 1 - s); }
-constexpr int ns_rev(int n, int s) {
+constexpr int radix_rev(int n, int s, int pl = 0) { return radix_of(n, num_stages(n, pl) - 1 - s, pl); }
+constexpr int ns_rev(int n, int s, int pl = 0) {
   int p = 1;
-  for (int i = 0; i < s; ++i) p *= radix_rev(n, i);
+  for (int i = 0; i < s; ++i) p *= radix_rev(n, i, pl);
   return p;
 }
 template <bool REV>
-constexpr int plan_radix(int n, int s) { return REV ? radix_rev(n, s) : radix_of(n, s); }
+constexpr int plan_radix(int n, int s, int pl = 0) { return REV ? radix_rev(n, s, pl) : radix_of(n, s, pl); }
 template <bool REV>
-constexpr int plan_ns(int n, int s) { return REV ? ns_rev(n, s) : ns_of(n, s); }
+constexpr int plan_ns(int n, int s, int pl = 0) { return REV ? ns_rev(n, s, pl) : ns_of(n, s, pl); }
 
 // Loaders take (t, a) or (t, a, slot), slot = it * R + r as for stores.
 template <typename Load>
@@ -215,10 +220,10 @@ __device__ __forceinline__ void run(const float2* tw, float sign, int cnt, Load&
 // work item's inputs are loaded into registers, then a barrier, then the
 // butterflies and stores (a Stockham stage writes other positions than it
 // reads, so without the barrier a store could overwrite an unread input).
-template <int N, int S, int CMAX, int NT, bool TFAST, bool REV = false, typename Load, typename Store>
+template <int N, int S, int CMAX, int NT, bool TFAST, bool REV = false, int PL = 0, typename Load, typename Store>
 __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld, Store& st) {
-  constexpr int R = plan_radix<REV>(N, S);
-  constexpr int NS = plan_ns<REV>(N, S);
+  constexpr int R = plan_radix<REV>(N, S, PL);
+  constexpr int NS = plan_ns<REV>(N, S, PL);
   constexpr int NB = N / R;
   constexpr int TMAX = NB * CMAX;
   constexpr int TWSTEP = N / (NS * R);
@@ -273,35 +278,35 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
 // stage 0 loads through `first`, the last stage stores through `last`, the
 // others read and write L.  Stages are separated by barriers; there is no
 // trailing barrier.
-template <int N, int CMAX, int NT, bool TFAST, bool REV = false, int S = 0, typename Load, typename Store,
-          typename Lds>
+template <int N, int CMAX, int NT, bool TFAST, bool REV = false, int PL = 0, int S = 0, typename Load,
+          typename Store, typename Lds>
 __device__ __forceinline__ void run_ip(const float2* tw, float sign, Load& first, Store& last, Lds L) {
-  constexpr int NSTAGES = num_stages(N);
+  constexpr int NSTAGES = num_stages(N, PL);
   static_assert(NSTAGES > 0, "length must factor into 2, 3, 5, 7");
   if constexpr (NSTAGES == 1) {
-    stage_ip<N, 0, CMAX, NT, TFAST, REV>(tw, sign, first, last);
+    stage_ip<N, 0, CMAX, NT, TFAST, REV, PL>(tw, sign, first, last);
   } else if constexpr (S == 0) {
     auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
-    stage_ip<N, 0, CMAX, NT, TFAST, REV>(tw, sign, first, st);
+    stage_ip<N, 0, CMAX, NT, TFAST, REV, PL>(tw, sign, first, st);
     __syncthreads();
-    run_ip<N, CMAX, NT, TFAST, REV, 1>(tw, sign, first, last, L);
+    run_ip<N, CMAX, NT, TFAST, REV, PL, 1>(tw, sign, first, last, L);
   } else if constexpr (S == NSTAGES - 1) {
     auto ld = [&](int t, int a) { return L(t, a); };
-    stage_ip<N, S, CMAX, NT, TFAST, REV>(tw, sign, ld, last);
+    stage_ip<N, S, CMAX, NT, TFAST, REV, PL>(tw, sign, ld, last);
   } else {
     auto ld = [&](int t, int a) { return L(t, a); };
     auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
-    stage_ip<N, S, CMAX, NT, TFAST, REV>(tw, sign, ld, st);
+    stage_ip<N, S, CMAX, NT, TFAST, REV, PL>(tw, sign, ld, st);
     __syncthreads();
-    run_ip<N, CMAX, NT, TFAST, REV, S + 1>(tw, sign, first, last, L);
+    run_ip<N, CMAX, NT, TFAST, REV, PL, S + 1>(tw, sign, first, last, L);
   }
 }
 
 // Registers a thread holds between a forward run_ip (last stage stores by
 // slot) and the reversed-plan inverse (first stage loads by slot).
-template <int N, int CMAX, int NT>
+template <int N, int CMAX, int NT, int PL = 0>
 constexpr int handoff_slots() {
-  constexpr int R = radix_of(N, num_stages(N) - 1);
+  constexpr int R = radix_of(N, num_stages(N, PL) - 1, PL);
   return ((N / R) * CMAX + NT - 1) / NT * R;
 }
 

@@ -759,13 +759,13 @@ bool fast_length(int n) {
 #define KW_BY_NT_PENCIL(nt, fn, ...) ((nt) == 512 ? fn<512>(__VA_ARGS__) : KW_BY_NT(nt, fn, __VA_ARGS__))
 
 static bool ypass_fast(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign,
-                       int nt) {
-  return KW_BY_NT_PENCIL(nt, launch_ypass_fast_t, s, g, n, tw, f, nf, sign);
+                       int nt, int plan) {
+  return KW_BY_NT_PENCIL(nt, launch_ypass_fast_t, s, g, n, tw, f, nf, sign, plan);
 }
 
 static bool last_fast(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T,
-                      int nt) {
-  return KW_BY_NT_PENCIL(nt, launch_last_fast_t, s, g, n, tw, a, T);
+                      int nt, int plan) {
+  return KW_BY_NT_PENCIL(nt, launch_last_fast_t, s, g, n, tw, a, T, plan);
 }
 
 static size_t x_smem(const Grid& g, int P, int nbuf, bool slot) {
@@ -778,7 +778,7 @@ static inline unsigned x_blocks(const Grid& g, int P) { return (unsigned)((g.row
 
 void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, float2* const* f, int nf,
                   float sign, TileLaunch tl) {
-  if (ypass_fast(s, g, ax.n, tw, f, nf, sign, tl.nt)) return;
+  if (ypass_fast(s, g, ax.n, tw, f, nf, sign, tl.nt, tl.plan)) return;
   Fields3 F{{f[0], nf > 1 ? f[1] : nullptr, nf > 2 ? f[2] : nullptr}};
   dim3 grid((g.Nxh + tl.tx - 1) / tl.tx, g.Nz, nf);
   size_t sm = (size_t)g.Ny * sizeof(float2) * (1 + 2 * tl.tx);
@@ -787,7 +787,7 @@ void launch_ypass(hipStream_t s, const Grid& g, const FftAxis& ax, const float2*
 
 void launch_last(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const LastArgs& a,
                  const SpecTables& T, TileLaunch tl) {
-  if (last_fast(s, g, ax.n, tw, a, T, tl.nt)) return;
+  if (last_fast(s, g, ax.n, tw, a, T, tl.nt, tl.plan)) return;
   const int Q = g.dim == 3 ? g.Nxh * g.Ny : g.Nxh;
   dim3 grid((Q + tl.tx - 1) / tl.tx);
   size_t sm = (size_t)ax.n * sizeof(float2) * (1 + 3 * tl.tx);

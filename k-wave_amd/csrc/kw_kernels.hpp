@@ -108,7 +108,8 @@ struct XLaunch {
 struct TileLaunch {
   int tx;       // pencils per workgroup (generic kernels)
   int threads;
-  int nt;       // workgroup size of the length-specialised kernels (64/128/256; 0 = generic only)
+  int nt;       // workgroup size of the length-specialised kernels (64/128/256/512; 0 = generic only)
+  int plan;     // radix plan of the length-specialised kernels (kwt::next_radix; 1 only for lengths >= 160)
 };
 
 // Axis lengths with compiled length-specialised kernels (kw_pencil_nt*.hip,
@@ -119,14 +120,18 @@ bool fast_length(int n);
 
 // plane pass (kw_plane.hip): LDS bytes for a (Ny, Nz) plane, 0 if none
 size_t plane_lds_bytes(int ny, int nz);
+// its workgroup size when none is requested (KW_PLANE_NT or the per-size default)
+int default_plane_nt(int ny, int nz);
 // nt: workgroup size 256, 512 or 1024; 0 picks the per-size default
 bool launch_plane(hipStream_t s, const Grid& g, const float2* twY, const float2* twZ, const LastArgs& a,
                   const SpecTables& T, int nt = 0);
 
 template <int NT>
-bool launch_ypass_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign);
+bool launch_ypass_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign,
+                         int plan);
 template <int NT>
-bool launch_last_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T);
+bool launch_last_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T,
+                        int plan);
 template <int NT>
 bool launch_x_velocity_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
                               float* const* u, const Src* src, const Sens* sens, int init);

@@ -284,7 +284,7 @@ TileLaunch choose_tile(int width, int n, int nbuf) {
   while (txmax > 1 && (size_t)n * sizeof(float2) * (1 + (size_t)nbuf * txmax) > cap) --txmax;
   int ntiles = (width + txmax - 1) / txmax;
   int tx = (width + ntiles - 1) / ntiles;
-  return TileLaunch{tx, 256, 0};
+  return TileLaunch{tx, 256, 0, 0};
 }
 
 // Thread count for the length-specialised kernels: 64, 128 or 256, or 0 to
@@ -294,6 +294,14 @@ int env_nt(const char* name, int def, bool allow512 = false) {
   if (!v || !*v) return def;
   const int n = std::atoi(v);
   return (n == 0 || n == 64 || n == 128 || n == 256 || (allow512 && n == 512)) ? n : def;
+}
+
+// Radix plan of a pencil length from the environment (0 or 1; plan 1 is
+// compiled for lengths >= 160 only), default 0.
+int env_plan(const char* name, int n) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return 0;
+  return (std::atoi(v) == 1 && n >= 160) ? 1 : 0;
 }
 
 size_t x_smem_bytes(const Grid& g, int P, int nbuf, bool slot) {
@@ -732,7 +740,7 @@ struct TuneKey {
   }
 };
 struct TuneCfg {
-  int row, pencil_y, pencil_last, plane;
+  int row, pencil_y, pencil_last, plane, plan_y, plan_last;
   double ms;
 };
 // One entry per key; the first caller tunes while holding the entry's lock, so
@@ -756,6 +764,12 @@ void apply_cfg(kw_sim* s, const TuneCfg& c) {
   if (s->tly.nt) s->tly.nt = c.pencil_y;
   if (s->tll.nt) s->tll.nt = c.pencil_last;
   s->plane_nt = c.plane;
+  s->tly.plan = c.plan_y;
+  s->tll.plan = c.plan_last;
+}
+
+TuneCfg current_cfg(const kw_sim* s) {
+  return TuneCfg{s->xl.nt, s->tly.nt, s->tll.nt, s->plane_nt, s->tly.plan, s->tll.plan, 1e30};
 }
 
 // back to step 0: the fields carried between steps and every output cleared
@@ -1051,6 +1065,9 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     s->xl.nt = fast_length(g.Nx) ? row_nt : 0;
     s->tly.nt = (g.dim == 3 && fast_length(g.Ny)) ? pencil_nt : 0;
     s->tll.nt = fast_length(s->axL.n) ? last_nt : 0;
+    // radix plans of the pencils (kwt::next_radix; plan 1 exists for lengths >= 160)
+    s->tly.plan = env_plan("KW_YPASS_PLAN", g.Ny);
+    s->tll.plan = env_plan("KW_LAST_PLAN", s->axL.n);
     if ((d->record_flags & KW_REC_U_NON_STAGGERED) && !s->xl.nt) {
       delete s;
       return fail(KW_ERR_UNSUPPORTED, "u_non_staggered needs Nx among the compiled row lengths "
@@ -1528,7 +1545,7 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
   try {
     HIPCHK(hipSetDevice(s->device));
     const TuneKey key = tune_key(s);
-    TuneCfg best{s->xl.nt, s->tly.nt, s->tll.nt, s->plane_nt, 1e30};
+    TuneCfg best = current_cfg(s);
     std::shared_ptr<TuneEntry> ent;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1541,33 +1558,51 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
     if (!ent->done) {
       if (!s->ev0) HIPCHK(hipEventCreate(&s->ev0));
       if (!s->ev1) HIPCHK(hipEventCreate(&s->ev1));
-      std::vector<int> rows{0};
-      if (s->xl.nt) rows = s->g.Nx <= 96 ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256};
-      // (y pass, last axis) workgroup sizes
-      std::vector<std::pair<int, int>> pencils{{s->tly.nt, s->tll.nt}};
-      // (axisymmetric grids: the radial pass replaces the pencils)
-      if (!s->g.kxm && !s->axisym && (s->tly.nt || s->tll.nt))
-        pencils = {{128, 128}, {256, 256}, {256, 512}, {512, 512}};
-      std::vector<int> planes{0};
-      if (s->g.kxm)
-        planes = s->g.Ny * s->g.Nz < 16384 ? std::vector<int>{256, 512, 1024} : std::vector<int>{512, 1024};
       const long long steps = std::min<long long>(s->Nt, trial_steps + 1);
-      for (int r : rows)
-        for (const auto& pe : pencils)
-          for (int pl : planes) {
-            TuneCfg c{r, pe.first, pe.second, pl, 0.0};
-            apply_cfg(s, c);
-            reset_state(s);
-            step(s, s->stream, 1, false);  // first step (code objects, caches)
-            HIPCHK(hipEventRecord(s->ev0, s->stream));
-            for (long long t = 2; t <= steps; ++t) step(s, s->stream, t, false);
-            HIPCHK(hipEventRecord(s->ev1, s->stream));
-            HIPCHK(hipEventSynchronize(s->ev1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-            c.ms = ms / std::max<long long>(1, steps - 1);
-            if (c.ms < best.ms) best = c;
-          }
+      auto trial = [&](TuneCfg c) {
+        apply_cfg(s, c);
+        reset_state(s);
+        step(s, s->stream, 1, false);  // first step (code objects, caches)
+        HIPCHK(hipEventRecord(s->ev0, s->stream));
+        for (long long t = 2; t <= steps; ++t) step(s, s->stream, t, false);
+        HIPCHK(hipEventRecord(s->ev1, s->stream));
+        HIPCHK(hipEventSynchronize(s->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        c.ms = ms / std::max<long long>(1, steps - 1);
+        if (c.ms < best.ms) best = c;
+      };
+      // coordinate descent from the defaults: row workgroup size, then
+      // (y pass, last axis) workgroup sizes x radix plans, then the plane pass
+      best.ms = 1e30;
+      trial(best);
+      if (s->xl.nt)
+        for (int r : (s->g.Nx <= 96 ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256})) {
+          TuneCfg c = best;
+          c.row = r;
+          trial(c);
+        }
+      // (axisymmetric grids: the radial pass replaces the pencils)
+      if (!s->g.kxm && !s->axisym && (s->tly.nt || s->tll.nt)) {
+        const std::vector<std::pair<int, int>> nts{{128, 128}, {256, 256}, {256, 512}, {512, 512}};
+        const bool py = s->g.dim == 3 && s->axY.n >= 160, pl = s->axL.n >= 160;
+        for (const auto& pe : nts)
+          for (int a = 0; a <= (py ? 1 : 0); ++a)
+            for (int b = 0; b <= (pl ? 1 : 0); ++b) {
+              TuneCfg c = best;
+              c.pencil_y = pe.first;
+              c.pencil_last = pe.second;
+              c.plan_y = a;
+              c.plan_last = b;
+              trial(c);
+            }
+      }
+      if (s->g.kxm)
+        for (int pn : (s->g.Ny * s->g.Nz < 16384 ? std::vector<int>{256, 512, 1024} : std::vector<int>{512, 1024})) {
+          TuneCfg c = best;
+          c.plane = pn;
+          trial(c);
+        }
       ent->cfg = best;
       ent->done = true;
     }
@@ -1579,6 +1614,19 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
   } catch (const HipError& e) {
     return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
   }
+  return KW_OK;
+}
+
+kw_status kw_sim_launch_config(const kw_sim* s, kw_launch_config* c) {
+  if (!s || !c) return fail(KW_ERR_INVALID, "null argument");
+  c->row_nt = s->xl.nt;
+  c->ypass_nt = s->g.dim == 3 && !s->g.kxm && !s->axisym ? s->tly.nt : 0;
+  c->last_nt = !s->g.kxm && !s->axisym ? s->tll.nt : 0;
+  c->plane_nt = s->g.kxm ? (s->plane_nt ? s->plane_nt : default_plane_nt(s->g.Ny, s->g.Nz)) : 0;
+  c->ypass_plan = c->ypass_nt ? s->tly.plan : 0;
+  c->last_plan = c->last_nt ? s->tll.plan : 0;
+  c->kx_major = s->g.kxm;
+  c->reserved = 0;
   return KW_OK;
 }
 

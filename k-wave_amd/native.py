@@ -40,6 +40,7 @@ EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
     "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
     "kw_sim_footprint_bytes", "kw_diag_copy", "kw_diag_copy_ex", "kw_sim_stream", "kw_sim_memory",
+    "kw_sim_launch_config",
     "kw_sim_checkpoint_bytes", "kw_sim_checkpoint_save", "kw_sim_checkpoint_load", "kw_sim_autotune",
     "kw_server_create", "kw_server_submit", "kw_server_poll", "kw_server_wait", "kw_server_request_info",
     "kw_server_request_error", "kw_server_output_bytes", "kw_server_fetch", "kw_server_release",
@@ -80,6 +81,12 @@ class KwSimDesc(C.Structure):
         ("transducer_source_input", C.POINTER(C.c_float)), ("delay_mask", C.POINTER(C.c_uint64)),
         ("axisymmetric_flag", C.c_uint64),
     ]
+
+
+class KwLaunchConfig(C.Structure):
+    _fields_ = [("row_nt", C.c_int32), ("ypass_nt", C.c_int32), ("last_nt", C.c_int32), ("plane_nt", C.c_int32),
+                ("ypass_plan", C.c_int32), ("last_plan", C.c_int32), ("kx_major", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class KwServerConfig(C.Structure):
@@ -146,6 +153,8 @@ def load_library(path: str = LIB_PATH):
     lib.kw_diag_copy_ex.restype = C.c_int
     lib.kw_sim_memory.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     lib.kw_sim_memory.restype = C.c_int
+    lib.kw_sim_launch_config.argtypes = [C.c_void_p, C.POINTER(KwLaunchConfig)]
+    lib.kw_sim_launch_config.restype = C.c_int
     vp, u64 = C.c_void_p, C.c_uint64
     for name, args, res in (
             ("kw_server_create", [C.POINTER(KwServerConfig), C.POINTER(vp)], C.c_int),
@@ -356,6 +365,12 @@ class Simulation:
         _check(self.lib, self.lib.kw_sim_autotune(self.handle, int(trial_steps), C.byref(r), C.byref(pe),
                                                   C.byref(pl)))
         return {"row_nt": r.value, "pencil_nt": pe.value, "plane_nt": pl.value}
+
+    def launch_config(self) -> dict:
+        """Workgroup sizes and pencil radix plans in use (kw_sim_launch_config)."""
+        c = KwLaunchConfig()
+        _check(self.lib, self.lib.kw_sim_launch_config(self.handle, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in KwLaunchConfig._fields_ if k != "reserved"}
 
     def memory(self) -> dict:
         """Device bytes held now and at most since creation (kw_sim_memory)."""

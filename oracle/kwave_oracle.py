@@ -476,17 +476,20 @@ def nonlinear_density(srho, rho0):
 
 
 def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: bool = False,
-        nsteps: int | None = None, timings: dict | None = None) -> dict:
+        nsteps: int | None = None, timings: dict | None = None, progress=None) -> dict:
     """Run the time loop; return outputs named and shaped like the output file.
 
     kspaceFirstOrder3D.m:767-1206 (2D.m:710-1042 for Nz == 1).
     ``record`` holds names from RECORD_NAMES; ``nsteps`` truncates the loop
     (the outputs then cover steps 1..nsteps).  ``timings`` (a dict) receives
     the wall time of every step ("step") and of its source-injection
-    branches ("source"), in seconds.
+    branches ("source"), in seconds.  ``progress(msg)``, if given, is called
+    after the set-up and after every step.
     """
     pb = decode(inp)
     ops = Operators(pb, full_complex)
+    if progress:
+        progress("operators ready")
     sp = Spectral(pb, full_complex)
     axs = AxisymOps(pb) if pb.axisymmetric else None
     dim, dt = pb.dim, pb.dt
@@ -683,6 +686,8 @@ def run(inp: dict, record=("p",), record_start_index: int = 1, full_complex: boo
         if timings is not None:
             timings["step"].append(time.perf_counter() - t_step0)
             timings["source"].append(t_src)
+        if progress:
+            progress(f"step {t_index} done ({time.perf_counter() - t_step0:.1f} s)")
     if "p_final" in rec:
         out["p_final"] = p.copy()
     if "u_final" in rec:

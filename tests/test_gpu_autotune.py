@@ -48,3 +48,37 @@ def test_server_with_autotune(kw):
         for f, rid in zip(reqs, ids):
             assert srv.wait(rid, 120.0) and srv.state(rid) == "done", srv.error(rid)
             assert cases.rel_l2(srv.fetch(rid, "p"), kw.run(f)["p"]) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(48, 160, 192), (32, 192, 160)])
+def test_radix_plans_agree(kw, oracle, monkeypatch, shape):
+    """Pencil radix plan 1 (radix 8 at most) against plan 0 and the oracle."""
+    f = cases.case_3d(N=shape, Nt=24, pml=4, source="p")
+    rec = ("p", "p_final")
+    ref = oracle.run(f, record=rec)
+    got = {}
+    for plan in (0, 1):
+        monkeypatch.setenv("KW_YPASS_PLAN", str(plan))
+        monkeypatch.setenv("KW_LAST_PLAN", str(plan))
+        with kw.Simulation(f, record=rec) as sim:
+            cfg = sim.launch_config()
+            print(f"[gpu autotune] {shape} plan {plan}: {cfg}", flush=True)
+            assert cfg["kx_major"] == 0 and cfg["ypass_plan"] == plan and cfg["last_plan"] == plan
+            sim.run()
+            got[plan] = sim.outputs()
+        for k in rec:
+            assert cases.rel_l2(got[plan][k], ref[k]) <= 1e-4, (plan, k)
+    for k in rec:
+        assert cases.rel_l2(got[1][k], got[0][k]) < 1e-5, k
+
+
+def test_autotune_reports_plans(kw):
+    f = cases.case_3d(N=(32, 160, 192), Nt=12, pml=4, source="p")
+    with kw.Simulation(f, record=("p",)) as sim:
+        before = sim.launch_config()
+        sim.autotune(trial_steps=2)
+        after = sim.launch_config()
+    print(f"[gpu autotune] default {before} tuned {after}", flush=True)
+    assert before["ypass_plan"] == 0 and before["last_plan"] == 0
+    assert after["ypass_plan"] in (0, 1) and after["last_plan"] in (0, 1)
+    assert after["ypass_nt"] in (128, 256, 512) and after["last_nt"] in (128, 256, 512)
