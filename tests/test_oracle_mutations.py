@@ -76,3 +76,53 @@ def test_wrong_non_staggered_shift_breaks_intensity_and_order(oracle, monkeypatc
     _fails(pins.test_plane_wave_intensity_from_u_non_staggered, oracle)
     if mistake == "sign":
         _fails(pins.test_nonstaggered_velocity_peaks_are_ordered, oracle)
+
+
+# ---- mistakes planted in a copy of the oracle's source ------------------------
+
+def _mutated(oracle, old, new):
+    """The oracle module with one statement replaced (a fresh module object)."""
+    import inspect
+    import types
+
+    import sys
+
+    src = inspect.getsource(oracle)
+    assert src.count(old) == 1, old
+    name = "kwave_oracle_mutant"
+    mod = types.ModuleType(name)
+    sys.modules[name] = mod  # (dataclasses look their module up by name)
+    try:
+        exec(compile(src.replace(old, new), name, "exec"), mod.__dict__)
+    finally:
+        del sys.modules[name]
+    return mod
+
+
+def test_dirichlet_p_into_one_component_breaks_pin(kw, oracle):
+    import test_frontend_pins as fp
+
+    m = _mutated(oracle, "rho[a].reshape(-1)[p_idx] = vals", "rho[0].reshape(-1)[p_idx] = vals * (a == 0)")
+    _fails(fp.test_dirichlet_pressure_source_sets_the_pressure, kw, m, 3)
+
+
+def test_additive_instead_of_dirichlet_u_breaks_pin(kw, oracle):
+    import test_frontend_pins as fp
+
+    m = _mutated(oracle, "                    uf[u_idx] = vals\n", "                    uf[u_idx] = uf[u_idx] + vals\n")
+    _fails(fp.test_dirichlet_velocity_source_sets_the_velocity, kw, m, 2)
+
+
+def test_running_max_as_min_breaks_all_outputs_pin(kw, oracle):
+    import test_frontend_pins as fp
+
+    m = _mutated(oracle, 'out["p_max_all"] = p.copy() if fi == 1 else np.maximum(out["p_max_all"], p)',
+                 'out["p_max_all"] = p.copy() if fi == 1 else np.minimum(out["p_max_all"], p)')
+    _fails(fp.test_all_and_final_outputs_equal_brute_force, kw, m, 2, 1)
+
+
+def test_final_field_from_first_step_breaks_all_outputs_pin(kw, oracle):
+    import test_frontend_pins as fp
+
+    m = _mutated(oracle, 'out["u" + nm + "_final"] = u[a].copy()', 'out["u" + nm + "_final"] = 0.0 * u[a]')
+    _fails(fp.test_all_and_final_outputs_equal_brute_force, kw, m, 3, 4)
