@@ -242,6 +242,35 @@ kw_status kw_diag_copy(uint64_t bytes, int device, double* ms);
 kw_status kw_diag_copy_ex(uint64_t bytes, int device, int variant, int blocks, double* ms);
 
 /* ---------------------------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one whole simulation -- kw_sim_create, kw_sim_run(Nt),
+ * kw_sim_fetch of each listed output, kw_sim_destroy -- with every device
+ * buffer (operators, state, work spectra, recorded outputs) inside a fixed
+ * scratch region that the library holds statically, KW_NOALLOC_SCRATCH_BYTES
+ * on each GPU it runs on, until the process ends.  It calls no hipMalloc,
+ * hipFree or other allocating routine; the work runs on `stream` (a
+ * hipStream_t; NULL is the null stream) and the call returns when it is done.
+ * Results equal kw_sim_create + kw_sim_run + kw_sim_fetch bit for bit.
+ * A description needing more than the region (the arena figure of its
+ * allocation sequence, with no early releases) fails with KW_ERR_UNSUPPORTED
+ * before any device work.  Calls on one GPU share its region: within a
+ * process they are serialised; the region keeps the last call's contents
+ * (everything that must start at zero is cleared by the call itself).
+ * ------------------------------------------------------------------------- */
+#define KW_NOALLOC_SCRATCH_BYTES (128ull << 20)
+
+typedef struct kw_output_buffer {
+  int32_t output_id; /* kw_output_id */
+  int32_t reserved;
+  void* host_dst;    /* kw_sim_output_bytes of that output */
+  size_t bytes;
+} kw_output_buffer;
+
+kw_status kw_sim_run_noalloc(const kw_sim_desc* desc, void* stream, const kw_output_buffer* outputs,
+                             int noutputs);
+
+/* ---------------------------------------------------------------------------
  * Serving loop.  The reference runs one simulation per process launch
  * (kspaceFirstOrder3DC.m:544, one system() call per kspaceFirstOrder3DC
  * call; kWaveTester.m:1936-1950 loops over many such calls).  A server keeps

@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 
-SOLVER_SOURCES = (["kw_kernels.hip", "kw_sim.cpp", "kw_server.cpp", "kw_plane.hip", "kw_axisym.hip"]
+SOLVER_SOURCES = (["kw_kernels.hip", "kw_sim.cpp", "kw_server.cpp", "kw_plane.hip", "kw_axisym.hip", "kw_noalloc.hip"]
                   + [f"kw_pencil_nt{n}.hip" for n in (64, 128, 256, 512)]
                   + [f"kw_xrow_nt{n}.hip" for n in (64, 128, 256)])
 IO_SOURCES = ["kw_io.cpp"]

@@ -40,7 +40,7 @@ EXPORTED = (
     "kw_sim_create", "kw_sim_run", "kw_sim_sync", "kw_sim_steps_done", "kw_sim_output_bytes",
     "kw_sim_fetch", "kw_sim_destroy", "kw_last_error", "kw_sim_set_profiling", "kw_sim_profile_get",
     "kw_sim_footprint_bytes", "kw_diag_copy", "kw_diag_copy_ex", "kw_sim_stream", "kw_sim_memory",
-    "kw_sim_launch_config",
+    "kw_sim_launch_config", "kw_sim_run_noalloc",
     "kw_sim_checkpoint_bytes", "kw_sim_checkpoint_save", "kw_sim_checkpoint_load", "kw_sim_autotune",
     "kw_server_create", "kw_server_submit", "kw_server_poll", "kw_server_wait", "kw_server_request_info",
     "kw_server_request_error", "kw_server_output_bytes", "kw_server_fetch", "kw_server_release",
@@ -87,6 +87,11 @@ class KwLaunchConfig(C.Structure):
     _fields_ = [("row_nt", C.c_int32), ("ypass_nt", C.c_int32), ("last_nt", C.c_int32), ("plane_nt", C.c_int32),
                 ("ypass_plan", C.c_int32), ("last_plan", C.c_int32), ("kx_major", C.c_int32),
                 ("reserved", C.c_int32)]
+
+
+class KwOutputBuffer(C.Structure):
+    _fields_ = [("output_id", C.c_int32), ("reserved", C.c_int32), ("host_dst", C.c_void_p),
+                ("bytes", C.c_size_t)]
 
 
 class KwServerConfig(C.Structure):
@@ -155,6 +160,8 @@ def load_library(path: str = LIB_PATH):
     lib.kw_sim_memory.restype = C.c_int
     lib.kw_sim_launch_config.argtypes = [C.c_void_p, C.POINTER(KwLaunchConfig)]
     lib.kw_sim_launch_config.restype = C.c_int
+    lib.kw_sim_run_noalloc.argtypes = [C.POINTER(KwSimDesc), C.c_void_p, C.POINTER(KwOutputBuffer), C.c_int]
+    lib.kw_sim_run_noalloc.restype = C.c_int
     vp, u64 = C.c_void_p, C.c_uint64
     for name, args, res in (
             ("kw_server_create", [C.POINTER(KwServerConfig), C.POINTER(vp)], C.c_int),
@@ -412,6 +419,33 @@ def run(f: dict, record=("p",), record_start_index: int = 1, device: int = 0) ->
     with Simulation(f, record, record_start_index, device) as sim:
         sim.run()
         return sim.outputs()
+
+
+def run_noalloc(f: dict, record=("p",), record_start_index: int = 1, device: int = 0,
+                stream: int | None = None) -> dict:
+    """A whole simulation through kw_sim_run_noalloc (every device buffer in
+    the library's static scratch region; no device allocation), returning
+    the same outputs as run()."""
+    lib = load_library()
+    d, keep = build_desc(f, record, record_start_index, device)
+    Nx, Ny, Nz = int(d.Nx), int(d.Ny), int(d.Nz)
+    dim = 3 if Nz > 1 else 2
+    ns = int(d.sensor_npoints)
+    nrec = int(d.Nt) - max(1, int(record_start_index)) + 1
+    names = output_names(record, dim)
+    outs, bufs = {}, (KwOutputBuffer * max(1, len(names)))()
+    for i, name in enumerate(names):
+        if _time_series(name):
+            a = np.empty((nrec, ns), dtype=np.float32)
+        elif name.endswith("_all") or name.endswith("_final"):
+            a = np.empty((Nz, Ny, Nx), dtype=np.float32)
+        else:
+            a = np.empty(ns, dtype=np.float32)
+        outs[name] = a
+        bufs[i] = KwOutputBuffer(output_id(name), 0, a.ctypes.data, a.nbytes)
+    _check(lib, lib.kw_sim_run_noalloc(C.byref(d), C.c_void_p(stream) if stream else None, bufs, len(names)))
+    del keep
+    return outs
 
 
 def footprint_bytes(f: dict, record=("p",), record_start_index: int = 1) -> int:
