@@ -295,7 +295,17 @@ typedef struct kw_server_config {
   uint32_t max_active;        /* simulations resident at once; 0: 8 */
   uint32_t steps_per_quantum; /* time steps queued per submission; 0: 16 */
   uint32_t autotune;          /* 1: kw_sim_autotune each admitted simulation (cached per shape) */
-  uint32_t reserved;
+  /* 1: batched mode.  Admitted 3-D simulations of one grid shape, physics
+   * branch and launch configuration advance together: each kernel of the time
+   * step is launched once for the whole group (one simulation per
+   * blockIdx.z, arguments in device tables, per-simulation time index kept
+   * on the device), and a quantum of steps is captured in a hipGraph and
+   * replayed while the group's membership is unchanged.  Simulations join
+   * and leave at quantum boundaries.  Results are the same as alone, bit for
+   * bit.  Simulations that cannot be batched (2-D, axisymmetric,
+   * u_non_staggered, additive sources with one signal per point, lengths
+   * without specialised kernels) run as in the default mode. */
+  uint32_t batching;
 } kw_server_config;
 
 typedef enum kw_request_state {

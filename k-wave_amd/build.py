@@ -32,6 +32,10 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off (they have given wrong results on these GPUs)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fuse only within one source expression, so the rounding of a
+    # kernel body does not depend on which kernel it is inlined into (the
+    # single-simulation and batched wrappers share every body)
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 

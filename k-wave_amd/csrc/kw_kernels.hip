@@ -794,6 +794,94 @@ void launch_last(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* 
   hipLaunchKernelGGL(k_last, grid, dim3(tl.threads), sm, s, g, ax, tw, a, T, tl.tx);
 }
 
+// ---- batched launches (length-specialised kernels only) ----------------------
+void BatchSink::kernel(const void* fn, dim3 grid, dim3 block, size_t smem, void** args) {
+  if (err != hipSuccess) return;
+  if (!graph) {
+    err = hipLaunchKernel(fn, grid, block, args, smem, stream);
+    return;
+  }
+  hipKernelNodeParams p{};
+  p.func = const_cast<void*>(fn);
+  p.gridDim = grid;
+  p.blockDim = block;
+  p.sharedMemBytes = (unsigned)smem;
+  p.kernelParams = args;
+  p.extra = nullptr;
+  hipGraphNode_t node = nullptr;
+  err = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &p);
+  if (err == hipSuccess) last = node;
+}
+
+bool launch_x_batch(BatchSink& s, const Grid& g, int which, const void* jobs, const int* live, int nb, bool two,
+                    bool slot, XLaunch xl) {
+  return KW_BY_NT(xl.nt, launch_x_batch_fast_t, s, g, which, jobs, live, nb, two, slot);
+}
+
+bool launch_ypass_batch(BatchSink& s, const Grid& g, int n, const YpassJob* J, const int* live, int nb, int nf,
+                        float sign, TileLaunch tl) {
+  return KW_BY_NT_PENCIL(tl.nt, launch_ypass_batch_fast_t, s, g, n, J, live, nb, nf, sign, tl.plan);
+}
+
+bool launch_last_batch(BatchSink& s, const Grid& g, int n, const LastJob* J, const int* live, int nb, int nout,
+                       TileLaunch tl) {
+  return KW_BY_NT_PENCIL(tl.nt, launch_last_batch_fast_t, s, g, n, J, live, nb, nout, tl.plan);
+}
+
+// Step prologue of a batch: advances each job's time index and rewrites the
+// per-step fields of its x-kernel arguments as the single-simulation host
+// loop does before each step (source activity, signal row and single-signal
+// value; sensor record flags and column; p_final at the last step); live[j]
+// is 0 once job j has run its Nt steps.
+__global__ void k_batch_clock(BatchClock* __restrict__ clk, XVelJob* __restrict__ vel, XDensJob* __restrict__ dens,
+                              XPresJob* __restrict__ pres, int* __restrict__ live, int nb) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  BatchClock& c = clk[j];
+  const long long t = c.t + 1;
+  const int on = t <= c.Nt;
+  live[j] = on;
+  if (!on) return;
+  c.t = t;
+  // pointers in the tables are offsets from each table array (PtrEncode)
+  auto src = [&](Src& S, int k, const PtrEncode& enc) {
+    S.active = c.flag[k] >= (unsigned long long)t;
+    S.sig = nullptr;
+    S.sval = 0.f;
+    if (S.active) {
+      const size_t row = (size_t)(t - 1) * c.nseries[k];
+      S.sig = c.sig[k] + row;
+      S.sval = c.single[k] ? c.sig[k][row] : 0.f;
+      enc(S.sig);
+    }
+  };
+  const bool rec = t >= c.start;
+  auto sens = [&](Sens& S, bool pressure, const PtrEncode& enc) {
+    S.record = rec;
+    S.first = rec && t == c.start;
+    S.t_rec = rec ? t - c.start : 0;
+    if (pressure) {
+      S.final_field = t == c.Nt ? c.p_final : nullptr;
+      enc(S.final_field);
+    }
+  };
+  const PtrEncode ev{reinterpret_cast<const char*>(vel)}, ed{reinterpret_cast<const char*>(dens)},
+      ep{reinterpret_cast<const char*>(pres)};
+  for (int k = 0; k < 3; ++k) {
+    src(vel[j].A.src[k], k, ev);
+    sens(vel[j].A.sens[k], false, ev);
+  }
+  src(dens[j].A.psrc, 3, ed);
+  sens(dens[j].A.ps, true, ed);
+  sens(pres[j].ps, true, ep);
+}
+
+void launch_batch_clock(BatchSink& s, BatchClock* clk, XVelJob* vel, XDensJob* dens, XPresJob* pres, int* live,
+                        int nb) {
+  void* args[] = {&clk, &vel, &dens, &pres, &live, &nb};
+  s.kernel(reinterpret_cast<const void*>(&k_batch_clock), dim3((nb + 63) / 64), dim3(64), 0, args);
+}
+
 void launch_x_velocity(hipStream_t s, const Grid& g, const FftAxis& ax, const float2* tw, const Medium& m,
                        float2* const* G, float* const* u, const Src* src, const Sens* sens, int init,
                        XLaunch xl) {

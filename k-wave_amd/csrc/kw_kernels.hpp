@@ -99,6 +99,229 @@ struct Fields3 {
   float2* f[3];
 };
 
+// Arguments of the x-row velocity and density kernels (kw_xrow.inc).
+struct VelArgsT {
+  float2* G[3];
+  float* u[3];
+  Src src[3];
+  Sens sens[3];
+  int init;
+};
+struct DensArgsT {
+  float2* G[3];
+  float* rho[3];
+  Src psrc;
+  float* srho;
+  float2* pk;
+  Sens ps;
+};
+
+// Batched launches: one simulation per blockIdx.z, each kernel's arguments
+// in a device table of one entry per simulation (all of one grid shape,
+// physics branch and launch configuration).  live[j] == 0 skips job j.
+struct XVelJob {
+  Grid g;
+  const float2* tw;
+  Medium m;
+  VelArgsT A;
+};
+struct XDensJob {
+  Grid g;
+  const float2* tw;
+  Medium m;
+  DensArgsT A;
+};
+struct XPresJob {
+  Grid g;
+  const float2* tw;
+  Medium m;
+  const float2* GA;
+  const float2* GB;
+  const float* srho;
+  float2* pk;
+  Sens ps;
+};
+struct PlaneJob {
+  Grid g;
+  const float2* twY;
+  const float2* twZ;
+  LastArgs A;
+  SpecTables T;
+};
+struct YpassJob {
+  Grid g;
+  const float2* tw;
+  Fields3 F;
+};
+struct LastJob {
+  Grid g;
+  const float2* tw;
+  LastArgs A;
+  SpecTables T;
+};
+
+// Pointer fields of the batch job tables hold byte offsets from the start
+// of the table array that contains them (null stays null): a batched kernel
+// rebases them on its table argument, so the compiler knows they address
+// global memory (pointers loaded from memory are otherwise generic and
+// every access through them a flat access).  visit_ptrs applies a functor to
+// every pointer field of an argument block.
+struct PtrEncode {
+  const char* base;
+  template <typename T>
+  __host__ __device__ void operator()(T*& p) const {
+    p = p ? (T*)(reinterpret_cast<const char*>(p) - base) : nullptr;
+  }
+};
+struct PtrDecode {
+  const char* base;
+  template <typename T>
+  __host__ __device__ void operator()(T*& p) const {
+    p = p ? (T*)(base + (long long)(size_t)p) : nullptr;
+  }
+};
+template <typename F>
+__host__ __device__ inline void visit_ptrs(SF& a, const F& f) { f(a.f); }
+template <typename F>
+__host__ __device__ inline void visit_ptrs(Src& s, const F& f) {
+  f(s.sig);
+  f(s.row_ptr);
+  f(s.lin);
+  f(s.pattern);
+  f(s.spec);
+  f(s.delay);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(Sens& s, const F& f) {
+  f(s.row_ptr);
+  f(s.lin);
+  f(s.raw);
+  f(s.vmax);
+  f(s.vmin);
+  f(s.rms_sum);
+  f(s.max_all);
+  f(s.min_all);
+  f(s.final_field);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(Medium& m, const F& f) {
+  visit_ptrs(m.c0, f);
+  visit_ptrs(m.rho0, f);
+  visit_ptrs(m.bona, f);
+  visit_ptrs(m.tau, f);
+  visit_ptrs(m.eta, f);
+  for (int i = 0; i < 3; ++i) {
+    visit_ptrs(m.rsg[i], f);
+    f(m.pml[i]);
+    f(m.pml_sg[i]);
+  }
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(VelArgsT& a, const F& f) {
+  for (int i = 0; i < 3; ++i) {
+    f(a.G[i]);
+    f(a.u[i]);
+    visit_ptrs(a.src[i], f);
+    visit_ptrs(a.sens[i], f);
+  }
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(DensArgsT& a, const F& f) {
+  for (int i = 0; i < 3; ++i) {
+    f(a.G[i]);
+    f(a.rho[i]);
+  }
+  visit_ptrs(a.psrc, f);
+  f(a.srho);
+  f(a.pk);
+  visit_ptrs(a.ps, f);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(LastArgs& a, const F& f) {
+  for (int i = 0; i < 3; ++i) {
+    f(a.in[i]);
+    f(a.out[i]);
+  }
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(SpecTables& t, const F& f) {
+  for (int i = 0; i < 3; ++i) {
+    f(t.dpos[i]);
+    f(t.dneg[i]);
+    f(t.shift[i]);
+  }
+  f(t.kappa);
+  f(t.skappa);
+  f(t.nabla1);
+  f(t.nabla2);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(Fields3& a, const F& f) {
+  for (int i = 0; i < 3; ++i) f(a.f[i]);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(XVelJob& j, const F& f) {
+  f(j.tw);
+  visit_ptrs(j.m, f);
+  visit_ptrs(j.A, f);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(XDensJob& j, const F& f) {
+  f(j.tw);
+  visit_ptrs(j.m, f);
+  visit_ptrs(j.A, f);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(XPresJob& j, const F& f) {
+  f(j.tw);
+  visit_ptrs(j.m, f);
+  f(j.GA);
+  f(j.GB);
+  f(j.srho);
+  f(j.pk);
+  visit_ptrs(j.ps, f);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(PlaneJob& j, const F& f) {
+  f(j.twY);
+  f(j.twZ);
+  visit_ptrs(j.A, f);
+  visit_ptrs(j.T, f);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(YpassJob& j, const F& f) {
+  f(j.tw);
+  visit_ptrs(j.F, f);
+}
+template <typename F>
+__host__ __device__ inline void visit_ptrs(LastJob& j, const F& f) {
+  f(j.tw);
+  visit_ptrs(j.A, f);
+  visit_ptrs(j.T, f);
+}
+
+// Where the batched launches go: launched on `stream`, or (graph set)
+// appended to an explicitly built hipGraph as a chain of kernel nodes (no
+// stream capture, so other threads' HIP calls cannot invalidate it).
+struct BatchSink {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+  hipError_t err = hipSuccess;
+  void kernel(const void* fn, dim3 grid, dim3 block, size_t smem, void** args);
+};
+
+// Per-job time index of a batch and what the step prologue derives the
+// per-step kernel arguments from (sources: ux, uy, uz, p).
+struct BatchClock {
+  long long t;  // steps done; advanced by the prologue
+  long long Nt, start;
+  const float* sig[4];  // signal bases (flag rows of nseries values)
+  unsigned long long flag[4], nseries[4];
+  int single[4];  // the step's value is passed as Src::sval
+  float* p_final;
+};
+
 // Host-side launch wrappers (kw_kernels.hip).
 struct XLaunch {
   int pairs;    // packed row pairs per workgroup (generic kernels)
@@ -126,12 +349,21 @@ int default_plane_nt(int ny, int nz);
 bool launch_plane(hipStream_t s, const Grid& g, const float2* twY, const float2* twZ, const LastArgs& a,
                   const SpecTables& T, int nt = 0);
 
+bool launch_plane_batch(BatchSink& s, const Grid& g, const PlaneJob* J, const int* live, int nb, int nout,
+                        int nt = 0);
+
 template <int NT>
 bool launch_ypass_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign,
                          int plan);
 template <int NT>
 bool launch_last_fast_t(hipStream_t s, const Grid& g, int n, const float2* tw, const LastArgs& a, const SpecTables& T,
                         int plan);
+template <int NT>
+bool launch_ypass_batch_fast_t(BatchSink& s, const Grid& g, int n, const YpassJob* J, const int* live, int nb,
+                               int nf, float sign, int plan);
+template <int NT>
+bool launch_last_batch_fast_t(BatchSink& s, const Grid& g, int n, const LastJob* J, const int* live, int nb, int nout,
+                              int plan);
 template <int NT>
 bool launch_x_velocity_fast_t(hipStream_t s, const Grid& g, const float2* tw, const Medium& m, float2* const* G,
                               float* const* u, const Src* src, const Sens* sens, int init);
@@ -149,6 +381,9 @@ bool launch_x_source_fast_t(hipStream_t s, const Grid& g, const float2* tw, cons
 template <int NT>
 bool launch_x_c2r_fast_t(hipStream_t s, const Grid& g, const float2* tw, const float2* in, float* out,
                          const float2* mult);
+template <int NT>
+bool launch_x_batch_fast_t(BatchSink& s, const Grid& g, int which, const void* jobs, const int* live, int nb,
+                           bool two, bool slot);
 template <int NT>
 bool launch_x_r2c_fast_t(hipStream_t s, const Grid& g, const float2* tw, const float* in, float2* out);
 
@@ -178,6 +413,18 @@ bool launch_x_c2r_mult(hipStream_t s, const Grid& g, const float2* tw, const flo
 bool launch_x_r2c(hipStream_t s, const Grid& g, const float2* tw, const float* in, float2* out, XLaunch xl);
 // out[j] = field[lin[j]], j < n
 void launch_gather(hipStream_t s, const float* field, const long long* lin, int n, float* out);
+
+// batched forms (one simulation per blockIdx.z; false: no length-specialised
+// kernel for this launch configuration).  which: 0 velocity, 1 density, 2
+// absorbing EOS.
+bool launch_x_batch(BatchSink& s, const Grid& g, int which, const void* jobs, const int* live, int nb, bool two,
+                    bool slot, XLaunch xl);
+bool launch_ypass_batch(BatchSink& s, const Grid& g, int n, const YpassJob* J, const int* live, int nb, int nf,
+                        float sign, TileLaunch tl);
+bool launch_last_batch(BatchSink& s, const Grid& g, int n, const LastJob* J, const int* live, int nb, int nout,
+                       TileLaunch tl);
+void launch_batch_clock(BatchSink& s, BatchClock* clk, XVelJob* vel, XDensJob* dens, XPresJob* pres, int* live,
+                        int nb);
 
 // Axisymmetric radial pass (kw_axisym.hip): tables at the odd frequencies
 // 2m+1 (m < 2 Ny) of the 4 Ny radially expanded grid

@@ -13,6 +13,10 @@
 // host memory, frees the device memory and returns the reservation.
 // Simulations on different streams overlap on the device; each one's kernels
 // and their order are those of a lone kw_sim, so its results are identical.
+// In batched mode (kw_server_config.batching) a runner hands its simulation
+// to one batch worker instead, which advances all simulations of a grid
+// shape and physics branch together: one launch per kernel of the step for
+// the whole group, a quantum of steps replayed from a captured hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +35,13 @@
 
 namespace kw {
 void set_last_error(const char* msg);
-}
+struct SimBatch;
+std::string sim_batch_key(const kw_sim* s);
+SimBatch* batch_create();
+void batch_destroy(SimBatch* b);
+kw_status batch_run(SimBatch* b, const std::vector<kw_sim*>& sims, int nsteps, hipStream_t st, bool use_graph);
+void sim_detach(kw_sim* s);
+}  // namespace kw
 
 namespace {
 
@@ -83,6 +93,136 @@ struct OwnedDesc {
   }
 };
 
+// A simulation handed to the batch worker by its runner.
+struct BatchEntry {
+  kw_sim* sim = nullptr;
+  std::string key;
+  uint64_t nt = 0;
+  bool done = false;
+  kw_status status = KW_OK;
+  std::string error;
+};
+
+// Batched mode: one worker advances every batchable simulation.  Members of
+// one batch key (grid shape, physics branch, launch configuration) form a
+// group with its own stream; each round launches one quantum of batched
+// steps per group (a replayed hipGraph once the group's membership has run
+// a quantum), then waits for the groups and hands finished simulations back
+// to their runners.  New members join at the next round.
+struct BatchWorker {
+  int device = 0;
+  unsigned quantum = 16;
+  std::mutex mu;
+  std::condition_variable cv;  // work arrived / an entry finished / stop
+  std::vector<std::shared_ptr<BatchEntry>> pending;
+  struct Group {
+    kw::SimBatch* batch = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<std::shared_ptr<BatchEntry>> members;
+  };
+  std::map<std::string, Group> groups;
+  bool stopping = false;
+  std::thread th;
+
+  void loop();
+  // blocks until the worker has run the simulation to its Nt (or failed)
+  kw_status run(kw_sim* sim, const std::string& key, uint64_t nt, std::string* msg) {
+    auto e = std::make_shared<BatchEntry>();
+    e->sim = sim;
+    e->key = key;
+    e->nt = nt;
+    std::unique_lock<std::mutex> lk(mu);
+    pending.push_back(e);
+    cv.notify_all();
+    cv.wait(lk, [&] { return e->done; });
+    *msg = e->error;
+    return e->status;
+  }
+};
+
+void BatchWorker::loop() {
+  (void)hipSetDevice(device);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      auto busy = [&] {
+        for (auto& g : groups)
+          if (!g.second.members.empty()) return true;
+        return false;
+      };
+      cv.wait(lk, [&] { return stopping || !pending.empty() || busy(); });
+      if (stopping && pending.empty() && !busy()) break;
+      for (auto& e : pending) {
+        Group& g = groups[e->key];
+        if (!g.batch) {
+          g.batch = kw::batch_create();
+          if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            g.stream = nullptr;
+          }
+        }
+        if (!g.stream) {
+          e->status = KW_ERR_DEVICE;
+          e->error = "hipStreamCreateWithFlags failed";
+          e->done = true;
+          continue;
+        }
+        g.members.push_back(e);
+      }
+      pending.clear();
+      cv.notify_all();
+    }
+    // one quantum per group, all queued before any wait so the groups overlap
+    std::vector<Group*> ran;
+    for (auto& kv : groups) {
+      Group& g = kv.second;
+      if (g.members.empty()) continue;
+      std::vector<kw_sim*> sims;
+      for (auto& m : g.members) sims.push_back(m->sim);
+      const kw_status st = kw::batch_run(g.batch, sims, (int)quantum, g.stream, true);
+      if (st != KW_OK) {
+        const std::string msg = kw_last_error();
+        (void)hipStreamSynchronize(g.stream);
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& m : g.members) {
+          kw::sim_detach(m->sim);
+          m->status = st;
+          m->error = msg;
+          m->done = true;
+        }
+        g.members.clear();
+        cv.notify_all();
+        continue;
+      }
+      ran.push_back(&g);
+    }
+    for (Group* g : ran) {
+      const hipError_t e = hipStreamSynchronize(g->stream);
+      std::lock_guard<std::mutex> lk(mu);
+      for (size_t i = 0; i < g->members.size();) {
+        BatchEntry& m = *g->members[i];
+        if (e != hipSuccess || kw_sim_steps_done(m.sim) >= m.nt) {
+          kw::sim_detach(m.sim);
+          if (e != hipSuccess) {
+            m.status = KW_ERR_DEVICE;
+            m.error = std::string("batched quantum failed: ") + hipGetErrorString(e);
+          }
+          m.done = true;
+          g->members.erase(g->members.begin() + (long)i);
+        } else {
+          ++i;
+        }
+      }
+      cv.notify_all();
+    }
+  }
+  for (auto& kv : groups) {
+    kw::batch_destroy(kv.second.batch);
+    if (kv.second.stream) (void)hipStreamDestroy(kv.second.stream);
+  }
+  groups.clear();
+}
+
 struct Request {
   uint64_t id = 0;
   std::unique_ptr<OwnedDesc> in;  // dropped once the simulation is on the device
@@ -101,6 +241,7 @@ struct kw_server {
   uint64_t budget = 0, reserved = 0, peak = 0;
   unsigned max_active = 8, quantum = 16;
   bool autotune = false;
+  std::unique_ptr<BatchWorker> batcher;  // batched mode
   Clock::time_point t0;
 
   std::mutex mu;
@@ -159,8 +300,29 @@ void kw_server::run_request(std::shared_ptr<Request> r) {
     std::lock_guard<std::mutex> lk(mu);
     r->in.reset();
   }
-  hipEvent_t ev[2] = {nullptr, nullptr};
   std::string msg;
+  const std::string bkey = batcher ? kw::sim_batch_key(sim) : std::string();
+  if (!bkey.empty()) {
+    // batched: the worker advances it with the other simulations of its key
+    st = batcher->run(sim, bkey, r->nt, &msg);
+    std::map<int, std::vector<unsigned char>> out;
+    for (int id = 0; id < KW_OUT_COUNT && st == KW_OK; ++id) {
+      const size_t b = kw_sim_output_bytes(sim, id);
+      if (!b) continue;
+      std::vector<unsigned char> h(b);
+      st = kw_sim_fetch(sim, id, h.data(), b);
+      if (st != KW_OK) msg = kw_last_error();
+      out.emplace(id, std::move(h));
+    }
+    kw_sim_destroy(sim);
+    if (st == KW_OK) {
+      std::lock_guard<std::mutex> lk(mu);
+      r->out = std::move(out);
+    }
+    finish(r, st, msg);
+    return;
+  }
+  hipEvent_t ev[2] = {nullptr, nullptr};
   for (auto& e : ev)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventBlockingSync) != hipSuccess) {
       st = KW_ERR_DEVICE;
@@ -267,6 +429,11 @@ kw_status kw_server_create(const kw_server_config* cfg, kw_server** out) {
   s->quantum = cfg->steps_per_quantum ? cfg->steps_per_quantum : 16;
   s->autotune = cfg->autotune != 0;
   if (hipSetDevice(s->device) != hipSuccess) return srv_fail(KW_ERR_DEVICE, "hipSetDevice failed");
+  if (cfg->batching) {
+    s->batcher = std::make_unique<BatchWorker>();
+    s->batcher->device = s->device;
+    s->batcher->quantum = s->quantum;
+  }
   s->budget = cfg->hbm_budget_bytes;
   if (!s->budget) {
     size_t fr = 0, tot = 0;
@@ -275,6 +442,7 @@ kw_status kw_server_create(const kw_server_config* cfg, kw_server** out) {
   }
   s->t0 = Clock::now();
   kw_server* p = s.release();
+  if (p->batcher) p->batcher->th = std::thread(&BatchWorker::loop, p->batcher.get());
   p->admitter = std::thread(&kw_server::admission_loop, p);
   *out = p;
   return KW_OK;
@@ -413,6 +581,15 @@ kw_status kw_server_destroy(kw_server* s) {
   s->admitter.join();
   // runners are only started by the admission thread, which has ended
   for (auto& t : s->runners) t.first.join();
+  if (s->batcher) {
+    // every runner has returned, so no simulation is left with the worker
+    {
+      std::lock_guard<std::mutex> lk(s->batcher->mu);
+      s->batcher->stopping = true;
+      s->batcher->cv.notify_all();
+    }
+    s->batcher->th.join();
+  }
   delete s;
   return KW_OK;
 }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -168,6 +169,7 @@ struct kw_sim {
   double* u_rms[3] = {};
   long long nrec_total = 0;  // Nt - start + 1
   bool own_stream = true;  // false: the caller's stream (kw_sim_run_noalloc)
+  uint64_t serial = 0;     // unique per handle in the process (batch membership)
   // profiling: an event pair per launch, resolved when the totals are read
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1028,6 +1030,10 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     }
   }
   kw_sim* s = new kw_sim();
+  {
+    static std::atomic<uint64_t> next_serial{0};
+    s->serial = ++next_serial;
+  }
   try {
     Grid& g = s->g;
     g.Nx = (int)d->Nx;
@@ -1695,7 +1701,415 @@ kw_status kw_diag_copy_ex(uint64_t bytes, int device, int variant, int blocks, d
 
 }  // extern "C"
 
+// =============================================================================
+// Batched steps: simulations of one grid shape, physics branch and launch
+// configuration advance together, each kernel of the step launched once for
+// the whole batch (one simulation per blockIdx.z) with per-simulation
+// arguments in device tables.  A step prologue kernel advances every
+// simulation's time index on the device, so the launches of a quantum of
+// steps do not change between quanta and are captured once in a hipGraph,
+// replayed while the batch's membership is unchanged.  Each simulation runs
+// the kernel bodies, launch shapes and statement order of its own
+// single-simulation step, so its results are the same bit for bit.
+// =============================================================================
+namespace {
+
+// The batch key (empty: the simulation's steps cannot be batched: 2-D and
+// axisymmetric grids, u_non_staggered, additive sources with one signal per
+// point, or lengths without specialised kernels).
+std::string batch_key_of(const kw_sim* s) {
+  const Grid& g = s->g;
+  if (g.dim != 3 || s->axisym || s->u_ns[0]) return std::string();
+  if (s->psrc.flag && s->psrc.mode == 2 && s->psrc.many) return std::string();
+  for (int c = 0; c < 3; ++c)
+    if (s->usrc[c].flag && s->usrc[c].mode == 2 && s->usrc[c].many) return std::string();
+  if (!s->xl.nt) return std::string();
+  if (!g.kxm && (!s->tly.nt || !s->tll.nt)) return std::string();
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%dx%dx%d kxm%d abs%d nl%d row%d y%d/%d/%d last%d/%d/%d plane%d dev%d", g.Nx,
+                g.Ny, g.Nz, g.kxm, s->m.absorbing, s->m.nonlinear, s->xl.nt, s->tly.nt, s->tly.plan, s->tly.tx,
+                s->tll.nt, s->tll.plan, s->tll.tx, s->plane_nt, s->device);
+  return std::string(buf);
+}
+
+}  // namespace
+
 namespace kw {
+
+struct SimBatch {
+  std::string key;
+  int device = 0;
+  // membership the device tables were built for, and each member's step
+  // count they assume
+  std::vector<kw_sim*> members;
+  std::vector<uint64_t> serials;
+  std::vector<long long> t_expect;
+  // device tables (one allocation)
+  char* table = nullptr;
+  size_t table_cap = 0;
+  hipStream_t table_stream = nullptr;
+  BatchClock* clk = nullptr;
+  int* live = nullptr;
+  XVelJob* vel = nullptr;
+  XDensJob* dens = nullptr;
+  XPresJob* pres = nullptr;
+  PlaneJob* plane[3] = {};
+  YpassJob* yin[3] = {};
+  LastJob* lastj[3] = {};
+  YpassJob* yout[3] = {};
+  int nstage = 0;
+  int stage_nin[3] = {}, stage_nout[3] = {};
+  bool slot_u = false, slot_p = false;
+  // host copies of the tables (kept until the next rebuild)
+  std::vector<char> host;
+  // captured quantum
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+  bool warmed = false;  // one direct quantum ran for this membership
+};
+
+}  // namespace kw
+
+namespace {
+
+size_t align_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
+
+// The spectral stages of a step (kspaceFirstOrder3D.m:790-801, 905-907,
+// 975-976/1001-1002), as step() builds them.
+int step_stages(const kw_sim* s, LastArgs* st) {
+  const Grid& g = s->g;
+  const float invN = (float)(1.0 / (double)g.N);
+  int n = 0;
+  {
+    LastArgs a{};
+    a.in[0] = s->Pk;
+    a.nin = 1;
+    a.nout = g.dim;
+    for (int c = 0; c < g.dim; ++c) {
+      a.out[c] = s->G[c];
+      a.outs[c] = SpecOut{0, OP_DPOS_K, c};
+    }
+    a.scale = invN;
+    st[n++] = a;
+  }
+  {
+    LastArgs a{};
+    a.nin = g.dim;
+    a.nout = g.dim;
+    for (int c = 0; c < g.dim; ++c) {
+      a.in[c] = s->G[c];
+      a.out[c] = s->G[c];
+      a.outs[c] = SpecOut{c, OP_DNEG_K, c};
+    }
+    a.scale = invN;
+    st[n++] = a;
+  }
+  if (s->m.absorbing == 1) {
+    LastArgs a{};
+    a.nin = 2;
+    a.nout = 2;
+    a.in[0] = a.out[0] = s->G[0];
+    a.in[1] = a.out[1] = s->G[1];
+    a.outs[0] = SpecOut{0, OP_NABLA1, 0};
+    a.outs[1] = SpecOut{1, OP_NABLA2, 0};
+    a.scale = invN;
+    st[n++] = a;
+  }
+  return n;
+}
+
+Fields3 fields_of(float2* const* f, int n) { return Fields3{{f[0], n > 1 ? f[1] : f[0], n > 2 ? f[2] : f[0]}}; }
+
+// Builds the device tables for the current members (host t as given).
+void batch_build(SimBatch* b, const std::vector<kw_sim*>& sims, hipStream_t st) {
+  const int nb = (int)sims.size();
+  const kw_sim* s0 = sims[0];
+  LastArgs stages0[3];
+  b->nstage = step_stages(s0, stages0);
+  for (int k = 0; k < b->nstage; ++k) {
+    b->stage_nin[k] = stages0[k].nin;
+    b->stage_nout[k] = stages0[k].nout;
+  }
+  const bool kxm = s0->g.kxm != 0;
+  // layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  const size_t o_clk = take(sizeof(BatchClock) * nb), o_live = take(sizeof(int) * nb);
+  const size_t o_vel = take(sizeof(XVelJob) * nb), o_dens = take(sizeof(XDensJob) * nb);
+  const size_t o_pres = take(sizeof(XPresJob) * nb);
+  size_t o_plane[3] = {}, o_yin[3] = {}, o_last[3] = {}, o_yout[3] = {};
+  for (int k = 0; k < b->nstage; ++k) {
+    if (kxm) {
+      o_plane[k] = take(sizeof(PlaneJob) * nb);
+    } else {
+      o_yin[k] = take(sizeof(YpassJob) * nb);
+      o_last[k] = take(sizeof(LastJob) * nb);
+      o_yout[k] = take(sizeof(YpassJob) * nb);
+    }
+  }
+  const size_t bytes = off;
+  if (bytes > b->table_cap) {
+    // stream-ordered: no device-wide synchronisation while other batches run
+    if (b->table) HIPCHK(hipFreeAsync(b->table, st));
+    b->table = nullptr;
+    const size_t cap = std::max(bytes, b->table_cap + b->table_cap / 2);
+    HIPCHK(hipMallocAsync((void**)&b->table, cap, st));
+    b->table_cap = cap;
+    b->table_stream = st;
+  }
+  b->host.assign(bytes, 0);
+  char* h = b->host.data();
+  auto* clk = reinterpret_cast<BatchClock*>(h + o_clk);
+  auto* vel = reinterpret_cast<XVelJob*>(h + o_vel);
+  auto* dens = reinterpret_cast<XDensJob*>(h + o_dens);
+  auto* pres = reinterpret_cast<XPresJob*>(h + o_pres);
+  b->slot_u = b->slot_p = false;
+  for (int j = 0; j < nb; ++j) {
+    kw_sim* s = sims[j];
+    const long long t1 = s->t + 1;  // the next step (the prologue rewrites the per-step fields)
+    BatchClock& c = clk[j];
+    c.t = s->t;
+    c.Nt = s->Nt;
+    c.start = s->start;
+    const SrcData* sd[4] = {&s->usrc[0], &s->usrc[1], &s->usrc[2], &s->psrc};
+    for (int k = 0; k < 4; ++k) {
+      c.sig[k] = sd[k]->sig;
+      c.flag[k] = sd[k]->flag;
+      c.nseries[k] = sd[k]->nseries;
+      c.single[k] = sd[k]->host_sig.empty() ? 0 : 1;
+    }
+    c.p_final = s->p_final;
+    Sens us[3] = {};
+    for (int a = 0; a < s->g.dim; ++a) us[a] = make_sens_u(s, t1, a);
+    XVelJob& v = vel[j];
+    v.g = s->g;
+    v.tw = s->twX;
+    v.m = s->m;
+    for (int a = 0; a < 3; ++a) {
+      v.A.G[a] = a < s->g.dim ? s->G[a] : nullptr;
+      v.A.u[a] = a < s->g.dim ? s->u[a] : nullptr;
+      v.A.src[a] = make_src(s, s->usrc[a], t1);
+      v.A.sens[a] = us[a];
+      if (s->usrc[a].flag && s->usrc[a].mode != 2) b->slot_u = true;
+    }
+    v.A.init = 0;
+    const Sens ps = make_sens_p(s, t1, t1 == s->Nt);
+    XDensJob& d = dens[j];
+    d.g = s->g;
+    d.tw = s->twX;
+    d.m = s->m;
+    for (int a = 0; a < 3; ++a) {
+      d.A.G[a] = a < s->g.dim ? s->G[a] : nullptr;
+      d.A.rho[a] = a < s->g.dim ? s->rho[a] : nullptr;
+    }
+    d.A.psrc = make_src(s, s->psrc, t1);
+    if (s->psrc.flag && s->psrc.mode != 2) b->slot_p = true;
+    d.A.srho = s->srho;
+    d.A.pk = s->Pk;
+    d.A.ps = ps;
+    XPresJob& pr = pres[j];
+    pr.g = s->g;
+    pr.tw = s->twX;
+    pr.m = s->m;
+    pr.GA = s->G[0];
+    pr.GB = s->G[1];
+    pr.srho = s->srho;
+    pr.pk = s->Pk;
+    pr.ps = ps;
+    LastArgs stages[3];
+    step_stages(s, stages);
+    for (int k = 0; k < b->nstage; ++k) {
+      if (kxm) {
+        reinterpret_cast<PlaneJob*>(h + o_plane[k])[j] = PlaneJob{s->g, s->twY, s->twL, stages[k], s->T};
+      } else {
+        reinterpret_cast<YpassJob*>(h + o_yin[k])[j] = YpassJob{s->g, s->twY, fields_of(stages[k].in, stages[k].nin)};
+        reinterpret_cast<LastJob*>(h + o_last[k])[j] = LastJob{s->g, s->twL, stages[k], s->T};
+        reinterpret_cast<YpassJob*>(h + o_yout[k])[j] =
+            YpassJob{s->g, s->twY, fields_of(stages[k].out, stages[k].nout)};
+      }
+    }
+  }
+  char* dv = b->table;
+  // pointers of the kernels' argument tables as offsets from their array
+  // (the batched kernels rebase them; the step prologue keeps the encoding)
+  for (int j = 0; j < nb; ++j) {
+    visit_ptrs(vel[j], PtrEncode{dv + o_vel});
+    visit_ptrs(dens[j], PtrEncode{dv + o_dens});
+    visit_ptrs(pres[j], PtrEncode{dv + o_pres});
+    for (int k = 0; k < b->nstage; ++k) {
+      if (kxm) {
+        visit_ptrs(reinterpret_cast<PlaneJob*>(h + o_plane[k])[j], PtrEncode{dv + o_plane[k]});
+      } else {
+        visit_ptrs(reinterpret_cast<YpassJob*>(h + o_yin[k])[j], PtrEncode{dv + o_yin[k]});
+        visit_ptrs(reinterpret_cast<LastJob*>(h + o_last[k])[j], PtrEncode{dv + o_last[k]});
+        visit_ptrs(reinterpret_cast<YpassJob*>(h + o_yout[k])[j], PtrEncode{dv + o_yout[k]});
+      }
+    }
+  }
+  b->clk = reinterpret_cast<BatchClock*>(dv + o_clk);
+  b->live = reinterpret_cast<int*>(dv + o_live);
+  b->vel = reinterpret_cast<XVelJob*>(dv + o_vel);
+  b->dens = reinterpret_cast<XDensJob*>(dv + o_dens);
+  b->pres = reinterpret_cast<XPresJob*>(dv + o_pres);
+  for (int k = 0; k < 3; ++k) {
+    b->plane[k] = k < b->nstage && kxm ? reinterpret_cast<PlaneJob*>(dv + o_plane[k]) : nullptr;
+    b->yin[k] = k < b->nstage && !kxm ? reinterpret_cast<YpassJob*>(dv + o_yin[k]) : nullptr;
+    b->lastj[k] = k < b->nstage && !kxm ? reinterpret_cast<LastJob*>(dv + o_last[k]) : nullptr;
+    b->yout[k] = k < b->nstage && !kxm ? reinterpret_cast<YpassJob*>(dv + o_yout[k]) : nullptr;
+  }
+  HIPCHK(hipMemcpyAsync(b->table, h, bytes, hipMemcpyHostToDevice, st));
+  b->members = sims;
+  b->serials.resize(sims.size());
+  b->t_expect.resize(sims.size());
+  for (size_t j = 0; j < sims.size(); ++j) {
+    b->serials[j] = sims[j]->serial;
+    b->t_expect[j] = sims[j]->t;
+  }
+}
+
+// One step of the batch: the prologue, then step()'s launch sequence for t >= 2
+// (direct launches, or kernel nodes of the quantum's graph).
+void batch_step(SimBatch* b, BatchSink& sk) {
+  const kw_sim* s0 = b->members[0];
+  const Grid& g = s0->g;
+  const int nb = (int)b->members.size();
+  launch_batch_clock(sk, b->clk, b->vel, b->dens, b->pres, b->live, nb);
+  auto stage = [&](int k) {
+    if (g.kxm) {
+      if (!launch_plane_batch(sk, g, b->plane[k], b->live, nb, b->stage_nout[k], s0->plane_nt))
+        throw HipError{hipErrorInvalidConfiguration, "batched plane pass"};
+      return;
+    }
+    if (!launch_ypass_batch(sk, g, s0->axY.n, b->yin[k], b->live, nb, b->stage_nin[k], -1.f, s0->tly) ||
+        !launch_last_batch(sk, g, s0->axL.n, b->lastj[k], b->live, nb, b->stage_nout[k], s0->tll) ||
+        !launch_ypass_batch(sk, g, s0->axY.n, b->yout[k], b->live, nb, b->stage_nout[k], 1.f, s0->tly))
+      throw HipError{hipErrorInvalidConfiguration, "batched pencil pass"};
+  };
+  stage(0);
+  if (!launch_x_batch(sk, g, 0, b->vel, b->live, nb, false, b->slot_u, s0->xl))
+    throw HipError{hipErrorInvalidConfiguration, "batched x velocity"};
+  stage(1);
+  const bool two = s0->m.absorbing == 1;
+  if (!launch_x_batch(sk, g, 1, b->dens, b->live, nb, two, b->slot_p, s0->xl))
+    throw HipError{hipErrorInvalidConfiguration, "batched x density"};
+  if (two) {
+    stage(2);
+    if (!launch_x_batch(sk, g, 2, b->pres, b->live, nb, false, false, s0->xl))
+      throw HipError{hipErrorInvalidConfiguration, "batched x pressure"};
+  }
+  HIPCHK(sk.err);
+}
+
+void batch_drop_graph(SimBatch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  b->graph_steps = 0;
+}
+
+}  // namespace
+
+namespace kw {
+
+std::string sim_batch_key(const kw_sim* s) { return s ? batch_key_of(s) : std::string(); }
+
+SimBatch* batch_create() { return new SimBatch(); }
+
+void batch_destroy(SimBatch* b) {
+  if (!b) return;
+  batch_drop_graph(b);
+  if (b->table) {
+    (void)hipFreeAsync(b->table, b->table_stream);
+    (void)hipStreamSynchronize(b->table_stream);
+  }
+  delete b;
+}
+
+// Advances every simulation of `sims` (one batch key, none finished) by up to
+// `nsteps` steps each (clamped to its Nt) on `st`.  The first step of a
+// simulation with an initial pressure runs on its own first.  use_graph:
+// replay a captured quantum once a membership has run one quantum directly.
+// Asynchronous; a finished simulation's outputs are complete once `st` is.
+kw_status batch_run(SimBatch* b, const std::vector<kw_sim*>& sims, int nsteps, hipStream_t st, bool use_graph) {
+  if (!b || sims.empty() || nsteps < 1) return KW_OK;
+  try {
+    HIPCHK(hipSetDevice(sims[0]->device));
+    const std::string key = batch_key_of(sims[0]);
+    if (key.empty()) return fail(KW_ERR_UNSUPPORTED, "simulation cannot run batched");
+    for (kw_sim* s : sims) {
+      if (batch_key_of(s) != key) return fail(KW_ERR_INVALID, "batch members differ in shape or configuration");
+      if (s->t >= s->Nt) return fail(KW_ERR_INVALID, "batch member has finished");
+    }
+    // the p0 start (3D.m:1021-1035) is a step of its own
+    for (kw_sim* s : sims)
+      if (s->t == 0 && s->p0) {
+        s->last = st;
+        step(s, st, 1, s->Nt == 1);
+        s->t = 1;
+      }
+    std::vector<kw_sim*> live;
+    for (kw_sim* s : sims)
+      if (s->t < s->Nt) live.push_back(s);
+    if (live.empty()) return KW_OK;
+    bool same = live == b->members;
+    for (size_t j = 0; same && j < live.size(); ++j)
+      same = live[j]->serial == b->serials[j] && live[j]->t == b->t_expect[j];
+    if (!same) {
+      batch_drop_graph(b);
+      b->warmed = false;
+      batch_build(b, live, st);
+    }
+    long long q = 0;
+    for (kw_sim* s : live) q = std::max<long long>(q, std::min<long long>(nsteps, s->Nt - s->t));
+    const int steps = (int)q;
+    if (use_graph && b->warmed) {
+      if (!b->exec || b->graph_steps != steps) {
+        // the quantum as an explicitly built chain of kernel nodes
+        batch_drop_graph(b);
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipGraphCreate(&graph, 0));
+        BatchSink sk;
+        sk.graph = graph;
+        try {
+          for (int i = 0; i < steps; ++i) batch_step(b, sk);
+        } catch (...) {
+          (void)hipGraphDestroy(graph);
+          throw;
+        }
+        const hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(e);
+        b->graph_steps = steps;
+      }
+      HIPCHK(hipGraphLaunch(b->exec, st));
+    } else {
+      BatchSink sk;
+      sk.stream = st;
+      for (int i = 0; i < steps; ++i) batch_step(b, sk);
+      b->warmed = true;
+    }
+    for (size_t j = 0; j < live.size(); ++j) {
+      kw_sim* s = live[j];
+      s->last = st;
+      s->t = std::min<long long>(s->Nt, s->t + steps);
+      b->t_expect[j] = s->t;
+    }
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    return fail(KW_ERR_DEVICE, "HIP error %d (%s) in %s", (int)e.e, hipGetErrorString(e.e), e.what);
+  }
+  return KW_OK;
+}
+
+// After the stream a simulation last ran on has been synchronised: later
+// calls on the handle wait on its own stream again.
+void sim_detach(kw_sim* s) {
+  if (s) s->last = nullptr;
+}
+
 // For kw_noalloc.hip: a simulation whose every device buffer lies in
 // [arena, arena + cap), run on `stream` (which may be the null stream).
 kw_status sim_create_in(const kw_sim_desc* d, void* arena, size_t cap, hipStream_t stream, kw_sim** out) {

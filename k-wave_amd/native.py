@@ -97,7 +97,7 @@ class KwOutputBuffer(C.Structure):
 class KwServerConfig(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
                 ("max_active", C.c_uint32), ("steps_per_quantum", C.c_uint32), ("autotune", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("batching", C.c_uint32)]
 
 
 class KwRequestInfo(C.Structure):
@@ -467,13 +467,15 @@ def diag_copy(nbytes: int, device: int = 0, variant: int = 0, blocks: int = 0) -
 
 class Server:
     """Serving loop (kw_server handle): submit many simulations, each admitted
-    by its device footprint and run concurrently with the others."""
+    by its device footprint and run concurrently with the others (batching:
+    simulations of one grid shape and physics branch share each kernel
+    launch, a quantum of steps replayed from a captured hipGraph)."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_active: int = 8,
-                 steps_per_quantum: int = 16, autotune: bool = False):
+                 steps_per_quantum: int = 16, autotune: bool = False, batching: bool = False):
         self.lib = load_library()
         cfg = KwServerConfig(KW_ABI_VERSION, device, int(hbm_budget_bytes), int(max_active),
-                             int(steps_per_quantum), 1 if autotune else 0, 0)
+                             int(steps_per_quantum), 1 if autotune else 0, 1 if batching else 0)
         h = C.c_void_p()
         _check(self.lib, self.lib.kw_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h
