@@ -22,9 +22,10 @@ Also reported, on the same line:
                  same workload (one step), extrapolated to Nt
   c2             the 128^3 configuration (plane-pass layout) as a second line
   serving        a sample of the serving stream: plain path (one simulation at
-                 a time) vs the serving loop, burst and Poisson arrivals at
-                 several loads, p50/p99 latency, and each mode's relative L2 /
-                 max-abs error against the FP64 oracle
+                 a time) vs the serving loop with concurrent streams and with
+                 batched launches (hipGraph quanta), burst and Poisson
+                 arrivals at several loads, p50/p99 latency, and each mode's
+                 relative L2 / max-abs error against the FP64 oracle
 
 Usage: python bench.py [--gpus 1] [--workload c4|c2|c3] [--steps K] [--warmup W]
                        [--no-cpu] [--serving-requests M] [--serving-rates 0.5,0.9,1.5]
@@ -319,9 +320,20 @@ def cpu_baseline(f: dict, rec, steps: int, nt_full: int) -> dict:
 # serving leg
 # ---------------------------------------------------------------------------
 
-def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int, autotune: bool = False):
+SERVING_MODES = {
+    # one simulation at a time: the reference's one-process-per-simulation loop, on the GPU
+    "plain": {"max_active": 1},
+    # concurrent simulations, each on its own stream (autotuned launch shapes)
+    "streams": {"max_active": 8, "autotune": True},
+    # simulations of one grid shape share each kernel launch; quanta replayed from hipGraphs
+    "batched": {"max_active": 32, "batching": True},
+}
+
+
+def _serve_stream(kw, reqs, arrivals, quantum: int, mode: str):
     """Submit reqs at the given arrival offsets (s) to a fresh server; wait for all."""
-    with kw.Server(max_active=max_active, steps_per_quantum=quantum, autotune=autotune) as srv:
+    cfg = dict(SERVING_MODES[mode])
+    with kw.Server(steps_per_quantum=quantum, **cfg) as srv:
         t0 = time.perf_counter()
         ids = []
         for f, ta in zip(reqs, arrivals):
@@ -340,7 +352,7 @@ def _serve_stream(kw, reqs, arrivals, max_active: int, quantum: int, autotune: b
         peak = srv.stats()["hbm_peak_reserved_bytes"]
     lat = np.array([i["t_done"] - i["t_submit"] for i in infos])
     upd = sum(_updates(f) for f in reqs)
-    res = {"max_active": max_active, "autotune": autotune, "makespan_s": round(makespan, 3),
+    res = {"mode": mode, **{k: v for k, v in cfg.items()}, "makespan_s": round(makespan, 3),
            "updates_per_s": upd / makespan,
            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
@@ -355,10 +367,12 @@ def _err(got, ref):
     return float(np.linalg.norm(g - r) / (nr if nr > 0 else 1.0)), float(np.max(np.abs(g - r)))
 
 
-def serving_leg(kw, M: int, rates, max_active: int, quantum: int, accuracy_requests: int) -> dict:
-    """Serving stream sample: plain (one simulation at a time) vs served
-    (continuous batching over streams) on the same requests, burst and Poisson
-    arrivals; each mode's error against the FP64 oracle on the smallest requests."""
+def serving_leg(kw, M: int, rates, quantum: int, accuracy_requests: int) -> dict:
+    """Serving stream sample: plain (one simulation at a time), streams
+    (concurrent simulations on their own streams) and batched (shared
+    launches, hipGraph quanta) on the same requests, burst and Poisson
+    arrivals; each mode's error against the FP64 oracle on the smallest
+    requests."""
     import concurrent.futures as cf
 
     from kwave_amd import configs
@@ -369,57 +383,59 @@ def serving_leg(kw, M: int, rates, max_active: int, quantum: int, accuracy_reque
     log(f"serving: generated {M} requests in {time.perf_counter() - t0:.1f} s")
     kw.run(configs.c5_request(0, grids=((32, 32, 32),), nt_range=(8, 8)))  # code objects loaded
     burst = [0.0] * M
-    plain, plain_out = _serve_stream(kw, reqs, burst, 1, quantum)
-    log(f"serving: plain burst {plain}")
-    served, served_out = _serve_stream(kw, reqs, burst, max_active, quantum, autotune=True)
-    log(f"serving: served burst {served}")
-    # served requests run autotuned launch shapes: equal to the plain path to FP32 rounding
-    rel = max(_err(b, a)[0] for a, b in zip(plain_out, served_out))
+    runs, outs = {}, {}
+    for mode in SERVING_MODES:
+        runs[mode], outs[mode] = _serve_stream(kw, reqs, burst, quantum, mode)
+        log(f"serving: {mode} burst {runs[mode]}")
+    plain = runs["plain"]
     grids = {}
     for f in reqs:
         g = "x".join(str(n) for n in _dims(f))
         grids[g] = grids.get(g, 0) + 1
     out = {"requests": M, "sample": f"c5_request(0..{M - 1}) of the seeded serving stream", "grids": grids,
            "updates": sum(_updates(f) for f in reqs), "steps_per_quantum": quantum,
-           "burst": {"plain": plain, "served": served,
-                     "speedup": round(served["updates_per_s"] / plain["updates_per_s"], 3)},
-           "max_rel_l2_served_vs_plain": rel, "matches_plain": bool(rel <= 1e-5), "poisson": []}
+           "burst": {**runs, **{f"speedup_{m}": round(runs[m]["updates_per_s"] / plain["updates_per_s"], 3)
+                                for m in SERVING_MODES if m != "plain"}},
+           # streams run autotuned launch shapes (equal to plain to FP32 rounding); batched runs
+           # the plain launch shapes and must equal plain bit for bit
+           "max_rel_l2_streams_vs_plain": max(_err(b, a)[0] for a, b in zip(outs["plain"], outs["streams"])),
+           "batched_bit_identical_to_plain": bool(all(np.array_equal(a, b)
+                                                      for a, b in zip(outs["plain"], outs["batched"]))),
+           "poisson": []}
     if accuracy_requests > 0:
         from oracle import kwave_oracle  # accuracy check only
 
         order = sorted(range(M), key=lambda i: (int(np.prod(_dims(reqs[i]))), _updates(reqs[i])))[:accuracy_requests]
-        acc = {"plain": {"max_rel_l2": 0.0, "max_abs": 0.0}, "served": {"max_rel_l2": 0.0, "max_abs": 0.0},
-               "requests": []}
+        acc = {m: {"max_rel_l2": 0.0, "max_abs": 0.0} for m in SERVING_MODES}
+        acc["requests"] = []
         for i in order:
             t1 = time.perf_counter()
             ref = kwave_oracle.run(reqs[i], record=("p",))["p"]
-            e_p = _err(plain_out[i], ref)
-            e_s = _err(served_out[i], ref)
-            for mode, e in (("plain", e_p), ("served", e_s)):
-                acc[mode]["max_rel_l2"] = max(acc[mode]["max_rel_l2"], e[0])
-                acc[mode]["max_abs"] = max(acc[mode]["max_abs"], e[1])
-            acc["requests"].append({"request": i, "grid": list(_dims(reqs[i])),
-                                    "Nt": int(np.asarray(reqs[i]["Nt"]).reshape(-1)[0]),
-                                    "peak_abs_p": float(np.max(np.abs(ref))),
-                                    "plain_rel_l2": e_p[0], "plain_max_abs": e_p[1],
-                                    "served_rel_l2": e_s[0], "served_max_abs": e_s[1]})
-            log(f"serving: request {i} vs FP64 oracle ({time.perf_counter() - t1:.1f} s): plain {e_p}, served {e_s}")
+            entry = {"request": i, "grid": list(_dims(reqs[i])), "Nt": int(np.asarray(reqs[i]["Nt"]).reshape(-1)[0]),
+                     "peak_abs_p": float(np.max(np.abs(ref)))}
+            for m in SERVING_MODES:
+                e = _err(outs[m][i], ref)
+                acc[m]["max_rel_l2"] = max(acc[m]["max_rel_l2"], e[0])
+                acc[m]["max_abs"] = max(acc[m]["max_abs"], e[1])
+                entry[f"{m}_rel_l2"], entry[f"{m}_max_abs"] = e
+            acc["requests"].append(entry)
+            log(f"serving: request {i} vs FP64 oracle ({time.perf_counter() - t1:.1f} s): {entry}")
         acc["tolerance_rel_l2"] = 1e-4
-        acc["within_tolerance"] = bool(max(acc["plain"]["max_rel_l2"], acc["served"]["max_rel_l2"]) <= 1e-4)
+        acc["within_tolerance"] = bool(max(acc[m]["max_rel_l2"] for m in SERVING_MODES) <= 1e-4)
         out["accuracy_vs_oracle"] = acc
-    del plain_out, served_out
+    del outs
     capacity = M / plain["makespan_s"]  # plain-path requests per second
     rng = np.random.default_rng(5999)
     gaps = rng.exponential(1.0, M)
     for r in rates:
         lam = r * capacity
         arr = list(np.cumsum(gaps / lam) - gaps[0] / lam)
-        p, _ = _serve_stream(kw, reqs, arr, 1, quantum)
-        s, _ = _serve_stream(kw, reqs, arr, max_active, quantum, autotune=True)
-        out["poisson"].append({"rate_per_s": round(lam, 3), "load_vs_plain_capacity": r, "plain": p, "served": s,
-                               "throughput_ratio": round(s["updates_per_s"] / p["updates_per_s"], 3)})
-        log(f"serving: poisson load {r}: plain p50/p99 {p['p50_latency_s']}/{p['p99_latency_s']} s, "
-            f"served {s['p50_latency_s']}/{s['p99_latency_s']} s")
+        res = {m: _serve_stream(kw, reqs, arr, quantum, m)[0] for m in SERVING_MODES}
+        out["poisson"].append({"rate_per_s": round(lam, 3), "load_vs_plain_capacity": r, **res,
+                               **{f"throughput_ratio_{m}": round(res[m]["updates_per_s"] / res["plain"]["updates_per_s"], 3)
+                                  for m in SERVING_MODES if m != "plain"}})
+        log(f"serving: poisson load {r}: " + ", ".join(
+            f"{m} p50/p99 {res[m]['p50_latency_s']}/{res[m]['p99_latency_s']} s" for m in SERVING_MODES))
     return out
 
 
@@ -442,7 +458,6 @@ def main():
                     help="serving-stream sample size (0 skips the serving leg)")
     ap.add_argument("--serving-rates", default="0.5,0.9,1.5",
                     help="Poisson arrival rates as fractions of the plain path's capacity")
-    ap.add_argument("--serving-max-active", type=int, default=8)
     ap.add_argument("--serving-quantum", type=int, default=16)
     ap.add_argument("--accuracy-requests", type=int, default=2,
                     help="smallest serving requests checked against the FP64 oracle (both modes)")
@@ -494,8 +509,8 @@ def main():
 
     if args.serving_requests > 0:
         rates = [float(x) for x in args.serving_rates.split(",") if x]
-        out["serving"] = serving_leg(kw, args.serving_requests, rates, args.serving_max_active,
-                                     args.serving_quantum, args.accuracy_requests)
+        out["serving"] = serving_leg(kw, args.serving_requests, rates, args.serving_quantum,
+                                     args.accuracy_requests)
 
     if not args.no_cpu:
         from kwave_amd import configs
