@@ -675,7 +675,7 @@ __global__ void k_pml(float* pml, int n, double d, double dt, double c, int size
 // nabla1 = k^(y-2), nabla2 = k^(y-1), Inf -> 0 (createAbsorptionVariables.m:34-48)
 __global__ void k_spectral_tables(Grid g, OpParams p, float* kappa, float* skappa, float* nabla1,
                                   float* nabla2) {
-  const long long nspec = (long long)g.Nxh * g.Ny * g.Nz;
+  const long long nspec = (long long)g.Nxs * g.Ny * g.Nz;
   const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nspec) return;
   int kxi, kyi, kzi;
@@ -684,11 +684,18 @@ __global__ void k_spectral_tables(Grid g, OpParams p, float* kappa, float* skapp
     const long long rest = id - (long long)kxi * g.Ny * g.Nz;
     kzi = (int)(rest / g.Ny);
     kyi = (int)(rest - (long long)kzi * g.Ny);
-  } else {      // [z][y][kx]
-    kxi = (int)(id % g.Nxh);
-    const long long rest = id / g.Nxh;
+  } else {      // [z][y][kx], row stride Nxs
+    kxi = (int)(id % g.Nxs);
+    const long long rest = id / g.Nxs;
     kyi = (int)(rest % g.Ny);
     kzi = (int)(rest / g.Ny);
+    if (kxi >= g.Nxh) {  // row padding: never read
+      if (kappa) kappa[id] = 0.f;
+      if (skappa) skappa[id] = 0.f;
+      if (nabla1) nabla1[id] = 0.f;
+      if (nabla2) nabla2[id] = 0.f;
+      return;
+    }
   }
   const double kx = kvec(kxi, g.Nx, p.dx[0]);
   const double ky = kvec(kyi, g.Ny, p.dx[1]);
@@ -974,7 +981,7 @@ void launch_pml(hipStream_t s, float* pml, int n, double d, double dt, double c,
 
 void launch_spectral_tables(hipStream_t s, const Grid& g, const OpParams& p, float* kappa, float* skappa,
                             float* nabla1, float* nabla2) {
-  const long long n = (long long)g.Nxh * g.Ny * g.Nz;
+  const long long n = (long long)g.Nxs * g.Ny * g.Nz;
   hipLaunchKernelGGL(k_spectral_tables, dim3(nblk(n, 256)), dim3(256), 0, s, g, p, kappa, skappa, nabla1, nabla2);
 }
 

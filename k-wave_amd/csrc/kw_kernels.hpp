@@ -18,6 +18,11 @@ struct Grid {
   // x-domain spectra layout: 0 row-major [z][y][kx] (kx fastest),
   // 1 kx-major [kx][z][y] (plane pass, y fastest)
   int kxm;
+  // row stride of row-major spectra (>= Nxh): padded to a multiple of 8
+  // complex values (64 bytes) on 3-D grids whose passes all have
+  // length-specialised kernels, so every pencil tile's row segment is whole
+  // 64-byte sectors; positions kx >= Nxh are never read or written
+  int Nxs;
 };
 
 // Scalar-or-field value: field pointer (nullptr for a scalar) + scalar.

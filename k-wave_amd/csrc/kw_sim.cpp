@@ -234,7 +234,7 @@ struct kw_sim {
   // callers that may capture the stream into a graph)
   void zero(void* p, size_t bytes) { launch_zero(stream, p, bytes); }
 
-  long long nspec() const { return (long long)g.Nxh * g.Ny * g.Nz; }
+  long long nspec() const { return (long long)g.Nxs * g.Ny * g.Nz; }
 };
 
 namespace {
@@ -304,6 +304,48 @@ int env_plan(const char* name, int n) {
   const char* v = std::getenv(name);
   if (!v || !*v) return 0;
   return (std::atoi(v) == 1 && n >= 160) ? 1 : 0;
+}
+
+// Layout of the x-domain spectra for a description: kx-major (plane pass)
+// when the (y, z) plane of one kx fits a workgroup's LDS and the x rows have
+// specialised kernels; otherwise row-major with the row stride Nxs (Grid::Nxs),
+// padded to a multiple of 8 complex values on 3-D grids whose passes all
+// have specialised kernels when that adds at most 6 % to the spectra
+// (KW_PAD_SPECTRA=0 never pads, =1 always may).  kw_sim_create and the
+// footprint formula both take it from here.
+struct SpecLayout {
+  int kxm, nxs;
+};
+SpecLayout spectra_layout(const kw_sim_desc* d, int lds_max) {
+  const int Nx = (int)d->Nx, Ny = (int)d->Ny, Nz = (int)d->Nz;
+  const int Nxh = Nx / 2 + 1, dim = Nz > 1 ? 3 : 2;
+  const int nlast = dim == 3 ? Nz : Ny;
+  const bool fast_rows = fast_length(Nx) && env_nt("KW_ROW_NT", 256) != 0;
+  const int pencil_nt = env_nt("KW_PENCIL_NT", 256, true);
+  const int last_nt = env_nt("KW_LAST_NT", std::getenv("KW_PENCIL_NT") ? pencil_nt : 256, true);
+  const bool fast_y = dim == 3 && fast_length(Ny) && pencil_nt != 0;
+  const bool fast_last = fast_length(nlast) && last_nt != 0;
+  const size_t need = dim == 3 ? plane_lds_bytes(Ny, Nz) : 0;
+  const char* pe = std::getenv("KW_PLANE");
+  const bool allow = !(pe && pe[0] == '0');
+  SpecLayout L{(allow && need > 0 && need <= (size_t)lds_max && fast_rows) ? 1 : 0, Nxh};
+  const char* pad = std::getenv("KW_PAD_SPECTRA");
+  if (dim == 3 && !d->axisymmetric_flag && !L.kxm && fast_rows && fast_y && fast_last && !(pad && pad[0] == '0')) {
+    const int nxs = (Nxh + 7) / 8 * 8;
+    if ((pad && pad[0] == '1') || (long long)(nxs - Nxh) * 100 <= 6LL * Nxh) L.nxs = nxs;
+  }
+  return L;
+}
+
+// The device's largest workgroup LDS allocation (gfx950: 160 KiB if the
+// device cannot be queried).
+int device_lds_max(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0) {
+    (void)hipGetLastError();
+    v = 160 * 1024;
+  }
+  return v;
 }
 
 size_t x_smem_bytes(const Grid& g, int P, int nbuf, bool slot) {
@@ -801,7 +843,7 @@ static bool footprint_plan(const kw_sim_desc* d, uint64_t* peak, uint64_t* total
   const uint64_t N = d->Nx * d->Ny * d->Nz;
   const int dim = d->Nz > 1 ? 3 : 2;
   const uint64_t Nxh = d->Nx / 2 + 1;
-  const uint64_t nspec = Nxh * d->Ny * d->Nz;
+  const uint64_t nspec = (uint64_t)spectra_layout(d, device_lds_max(d->device)).nxs * d->Ny * d->Nz;
   const uint64_t rows = d->Ny * d->Nz;
   const bool want_ns = (d->record_flags & KW_REC_U_NON_STAGGERED) != 0;
   const bool additive = (d->p_source_flag && d->p_source_mode == 2) ||
@@ -1040,6 +1082,7 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     g.Ny = (int)d->Ny;
     g.Nz = (int)d->Nz;
     g.Nxh = g.Nx / 2 + 1;
+    g.Nxs = g.Nxh;
     g.dim = d->Nz > 1 ? 3 : 2;
     g.rows = g.Ny * g.Nz;
     g.N = N;
@@ -1090,10 +1133,12 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     {
       int lds_max = 0;
       HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
-      const size_t need = g.dim == 3 ? plane_lds_bytes(g.Ny, g.Nz) : 0;
-      const char* pe = std::getenv("KW_PLANE");
-      const bool allow = !(pe && pe[0] == '0');
-      g.kxm = (allow && need > 0 && need <= (size_t)lds_max && s->xl.nt > 0) ? 1 : 0;
+      const SpecLayout L = spectra_layout(d, lds_max);
+      g.kxm = L.kxm;
+      g.Nxs = L.nxs;
+      // padded rows need every pass specialised (the generic kernels use Nxh rows)
+      if (g.Nxs != g.Nxh && !(s->xl.nt && s->tly.nt && s->tll.nt))
+        throw HipError{hipErrorInvalidConfiguration, "spectra_layout"};
       if (d->axisymmetric_flag) {
         s->axisym = 1;
         s->as_tx = make_axis(2 * g.Ny, &s->axR) ? as_radial_tile(g.Ny, (size_t)lds_max, &s->as_lds) : 0;
