@@ -367,7 +367,7 @@ def _err(got, ref):
     return float(np.linalg.norm(g - r) / (nr if nr > 0 else 1.0)), float(np.max(np.abs(g - r)))
 
 
-def serving_leg(kw, M: int, rates, quantum: int, accuracy_requests: int) -> dict:
+def serving_leg(kw, M: int, rates, quantum: int, accuracy_requests: int, small_requests: int = 0) -> dict:
     """Serving stream sample: plain (one simulation at a time), streams
     (concurrent simulations on their own streams) and batched (shared
     launches, hipGraph quanta) on the same requests, burst and Poisson
@@ -424,6 +424,18 @@ def serving_leg(kw, M: int, rates, quantum: int, accuracy_requests: int) -> dict
         acc["within_tolerance"] = bool(max(acc[m]["max_rel_l2"] for m in SERVING_MODES) <= 1e-4)
         out["accuracy_vs_oracle"] = acc
     del outs
+    if small_requests > 0:
+        # the serving stream's two smaller grid shapes only: grids that leave most of
+        # the GPU idle one at a time, where shared launches pay most
+        with cf.ThreadPoolExecutor(8) as ex:
+            sreqs = list(ex.map(lambda i: configs.c5_request(i, grids=configs.C5_GRIDS[:2]), range(small_requests)))
+        sb = {m: _serve_stream(kw, sreqs, [0.0] * small_requests, quantum, m)[0] for m in SERVING_MODES}
+        out["small_grid_burst"] = {"requests": small_requests,
+                                   "sample": f"c5_request(0..{small_requests - 1}) on the 64^3 and 96^3 grids",
+                                   "updates": sum(_updates(f) for f in sreqs), **sb,
+                                   **{f"speedup_{m}": round(sb[m]["updates_per_s"] / sb["plain"]["updates_per_s"], 3)
+                                      for m in SERVING_MODES if m != "plain"}}
+        log(f"serving: small-grid burst " + ", ".join(f"{m} {sb[m]['updates_per_s']:.3g}/s" for m in SERVING_MODES))
     capacity = M / plain["makespan_s"]  # plain-path requests per second
     rng = np.random.default_rng(5999)
     gaps = rng.exponential(1.0, M)
@@ -459,6 +471,8 @@ def main():
     ap.add_argument("--serving-rates", default="0.5,0.9,1.5",
                     help="Poisson arrival rates as fractions of the plain path's capacity")
     ap.add_argument("--serving-quantum", type=int, default=16)
+    ap.add_argument("--serving-small-requests", type=int, default=48,
+                    help="a second burst on the 64^3 and 96^3 serving grids only (0 skips)")
     ap.add_argument("--accuracy-requests", type=int, default=2,
                     help="smallest serving requests checked against the FP64 oracle (both modes)")
     args = ap.parse_args()
@@ -510,7 +524,7 @@ def main():
     if args.serving_requests > 0:
         rates = [float(x) for x in args.serving_rates.split(",") if x]
         out["serving"] = serving_leg(kw, args.serving_requests, rates, args.serving_quantum,
-                                     args.accuracy_requests)
+                                     args.accuracy_requests, args.serving_small_requests)
 
     if not args.no_cpu:
         from kwave_amd import configs

@@ -214,7 +214,7 @@ typedef struct kw_launch_config {
   int32_t row_nt, ypass_nt, last_nt, plane_nt;
   int32_t ypass_plan, last_plan;
   int32_t kx_major; /* 1: kx-major spectra and the plane pass */
-  int32_t reserved;
+  int32_t row_wg_target; /* x-row kernels: workgroups per launch the row pairs per workgroup aim at */
 } kw_launch_config;
 kw_status kw_sim_launch_config(const kw_sim* sim, kw_launch_config* config);
 

@@ -18,6 +18,10 @@ struct Grid {
   // x-domain spectra layout: 0 row-major [z][y][kx] (kx fastest),
   // 1 kx-major [kx][z][y] (plane pass, y fastest)
   int kxm;
+  // x-row kernels' target number of workgroups per launch (0: 1024 or
+  // KW_ROW_WGS): fewer row pairs per workgroup until a launch has this many;
+  // a launch-geometry choice of the autotuner, the results do not depend on it
+  int row_wgs;
   // row stride of row-major spectra (>= Nxh): padded to a multiple of 8
   // complex values (64 bytes) on 3-D grids whose passes all have
   // length-specialised kernels, so every pencil tile's row segment is whole

@@ -348,6 +348,13 @@ int device_lds_max(int device) {
   return v;
 }
 
+// The x-row kernels' default workgroup target (KW_ROW_WGS or 1024).
+long long row_wg_default() {
+  const char* e = std::getenv("KW_ROW_WGS");
+  const long long n = e ? std::atoll(e) : 0;
+  return n >= 64 ? n : 1024LL;
+}
+
 size_t x_smem_bytes(const Grid& g, int P, int nbuf, bool slot) {
   size_t s = (size_t)g.Nx * sizeof(float2) + (size_t)nbuf * P * g.Nx * sizeof(float2);
   if (slot) s += (size_t)2 * P * g.Nx * sizeof(int);
@@ -784,7 +791,7 @@ struct TuneKey {
   }
 };
 struct TuneCfg {
-  int row, pencil_y, pencil_last, plane, plan_y, plan_last;
+  int row, pencil_y, pencil_last, plane, plan_y, plan_last, row_wgs;
   double ms;
 };
 // One entry per key; the first caller tunes while holding the entry's lock, so
@@ -810,10 +817,11 @@ void apply_cfg(kw_sim* s, const TuneCfg& c) {
   s->plane_nt = c.plane;
   s->tly.plan = c.plan_y;
   s->tll.plan = c.plan_last;
+  s->g.row_wgs = c.row_wgs;
 }
 
 TuneCfg current_cfg(const kw_sim* s) {
-  return TuneCfg{s->xl.nt, s->tly.nt, s->tll.nt, s->plane_nt, s->tly.plan, s->tll.plan, 1e30};
+  return TuneCfg{s->xl.nt, s->tly.nt, s->tll.nt, s->plane_nt, s->tly.plan, s->tll.plan, s->g.row_wgs, 1e30};
 }
 
 // back to step 0: the fields carried between steps and every output cleared
@@ -1083,6 +1091,7 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     g.Nz = (int)d->Nz;
     g.Nxh = g.Nx / 2 + 1;
     g.Nxs = g.Nxh;
+    g.row_wgs = (int)row_wg_default();
     g.dim = d->Nz > 1 ? 3 : 2;
     g.rows = g.Ny * g.Nz;
     g.N = N;
@@ -1627,12 +1636,19 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
       // (y pass, last axis) workgroup sizes x radix plans, then the plane pass
       best.ms = 1e30;
       trial(best);
-      if (s->xl.nt)
+      if (s->xl.nt) {
         for (int r : (s->g.Nx <= 96 ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256})) {
           TuneCfg c = best;
           c.row = r;
           trial(c);
         }
+        // row pairs per workgroup, through the launch's workgroup target
+        for (int w : {512, 2048, 4096}) {
+          TuneCfg c = best;
+          c.row_wgs = w;
+          trial(c);
+        }
+      }
       // (axisymmetric grids: the radial pass replaces the pencils)
       if (!s->g.kxm && !s->axisym && (s->tly.nt || s->tll.nt)) {
         const std::vector<std::pair<int, int>> nts{{128, 128}, {256, 256}, {256, 512}, {512, 512}};
@@ -1677,7 +1693,7 @@ kw_status kw_sim_launch_config(const kw_sim* s, kw_launch_config* c) {
   c->ypass_plan = c->ypass_nt ? s->tly.plan : 0;
   c->last_plan = c->last_nt ? s->tll.plan : 0;
   c->kx_major = s->g.kxm;
-  c->reserved = 0;
+  c->row_wg_target = s->xl.nt ? s->g.row_wgs : 0;
   return KW_OK;
 }
 
@@ -1771,9 +1787,9 @@ std::string batch_key_of(const kw_sim* s) {
   if (!s->xl.nt) return std::string();
   if (!g.kxm && (!s->tly.nt || !s->tll.nt)) return std::string();
   char buf[256];
-  std::snprintf(buf, sizeof buf, "%dx%dx%d kxm%d abs%d nl%d row%d y%d/%d/%d last%d/%d/%d plane%d dev%d", g.Nx,
-                g.Ny, g.Nz, g.kxm, s->m.absorbing, s->m.nonlinear, s->xl.nt, s->tly.nt, s->tly.plan, s->tly.tx,
-                s->tll.nt, s->tll.plan, s->tll.tx, s->plane_nt, s->device);
+  std::snprintf(buf, sizeof buf, "%dx%dx%d/%d kxm%d abs%d nl%d row%d/%d y%d/%d/%d last%d/%d/%d plane%d dev%d", g.Nx,
+                g.Ny, g.Nz, g.Nxs, g.kxm, s->m.absorbing, s->m.nonlinear, s->xl.nt, g.row_wgs, s->tly.nt, s->tly.plan,
+                s->tly.tx, s->tll.nt, s->tll.plan, s->tll.tx, s->plane_nt, s->device);
   return std::string(buf);
 }
 

@@ -86,7 +86,7 @@ class KwSimDesc(C.Structure):
 class KwLaunchConfig(C.Structure):
     _fields_ = [("row_nt", C.c_int32), ("ypass_nt", C.c_int32), ("last_nt", C.c_int32), ("plane_nt", C.c_int32),
                 ("ypass_plan", C.c_int32), ("last_plan", C.c_int32), ("kx_major", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("row_wg_target", C.c_int32)]
 
 
 class KwOutputBuffer(C.Structure):
@@ -377,7 +377,7 @@ class Simulation:
         """Workgroup sizes and pencil radix plans in use (kw_sim_launch_config)."""
         c = KwLaunchConfig()
         _check(self.lib, self.lib.kw_sim_launch_config(self.handle, C.byref(c)))
-        return {k: getattr(c, k) for k, _ in KwLaunchConfig._fields_ if k != "reserved"}
+        return {k: getattr(c, k) for k, _ in KwLaunchConfig._fields_}
 
     def memory(self) -> dict:
         """Device bytes held now and at most since creation (kw_sim_memory)."""

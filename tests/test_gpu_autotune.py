@@ -82,3 +82,21 @@ def test_autotune_reports_plans(kw):
     assert before["ypass_plan"] == 0 and before["last_plan"] == 0
     assert after["ypass_plan"] in (0, 1) and after["last_plan"] in (0, 1)
     assert after["ypass_nt"] in (128, 256, 512) and after["last_nt"] in (128, 256, 512)
+    # the row kernels' workgroup target (pairs per workgroup) is a tuned launch shape too
+    assert before["row_wg_target"] == 1024 and after["row_wg_target"] in (512, 1024, 2048, 4096)
+
+
+def test_row_workgroup_target_leaves_results_unchanged(kw, monkeypatch):
+    """Row pairs per workgroup (through the workgroup target) change the launch
+    geometry only: a row's transform is the same, so results are bit-identical."""
+    f = cases.case_3d(N=(64, 48, 32), Nt=20, pml=4, source="uxyz")
+    outs = []
+    for wgs in ("256", "1024", "8192"):
+        monkeypatch.setenv("KW_ROW_WGS", wgs)
+        with kw.Simulation(f, record=("p", "u_max")) as sim:
+            assert sim.launch_config()["row_wg_target"] == int(wgs)
+            sim.run()
+            outs.append(sim.outputs())
+    for o in outs[1:]:
+        for k in o:
+            assert np.array_equal(o[k], outs[0][k]), k

@@ -225,14 +225,22 @@ def test_axisymmetric_time_reversal_through_the_contract(kw, oracle, tmp_path):
     assert cc > 0.99
 
 
-@pytest.mark.parametrize("how", ["abi", "binary"])
-def test_checkpoint_restart_is_exact(kw, tmp_path, how):
+@pytest.mark.parametrize("how,source", [("abi", "p_many"), ("binary", "p_many"), ("abi", "transducer"),
+                                        ("binary", "transducer")])
+def test_checkpoint_restart_is_exact(kw, tmp_path, how, source):
     """A run stopped at checkpoints and resumed (twice) gives bit-identical outputs
-    to the uninterrupted run; the finished run removes the checkpoint."""
+    to the uninterrupted run; the finished run removes the checkpoint.  The
+    transducer case restarts while its delayed signal is still being applied
+    (the reference binary's release notes record a wrong time-signal index on
+    restart for this source, k-wave_release_notes.html:215)."""
     from kwave_amd import h5format
 
-    print(f"[gpu binary] checkpoint-restart via {how}", flush=True)
-    f = cases.case_3d(N=(32, 28, 24), Nt=50, nonlinear=True, source="p", many=True)
+    print(f"[gpu binary] checkpoint-restart via {how}, {source} source", flush=True)
+    if source == "transducer":
+        f = cases.case_transducer(N=(32, 32, 32), Nt=50)
+        assert int(np.asarray(f["transducer_source_flag"]).reshape(-1)[0]) > 40
+    else:
+        f = cases.case_3d(N=(32, 28, 24), Nt=50, nonlinear=True, source="p", many=True)
     rec = ("p", "p_max", "p_rms", "u_max", "p_final", "u_non_staggered")
     inp = str(tmp_path / "in.h5")
     h5format.write_input(inp, f)
