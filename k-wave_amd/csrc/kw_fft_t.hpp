@@ -302,6 +302,82 @@ __device__ __forceinline__ void run_ip(const float2* tw, float sign, Load& first
   }
 }
 
+// In-place stage and transform over a runtime number of transforms cnt <=
+// CMAX, work items transform-major (t = w / NB), as stage()/run() distribute
+// them: the same slot numbering, so last_pos() maps the last stage's slots.
+template <int N, int S, int CMAX, int NT, typename Load, typename Store>
+__device__ __forceinline__ void stage_ipc(const float2* tw, float sign, int cnt, Load& ld, Store& st) {
+  constexpr int R = radix_of(N, S);
+  constexpr int NS = ns_of(N, S);
+  constexpr int NB = N / R;
+  constexpr int TMAX = NB * CMAX;
+  constexpr int TWSTEP = N / (NS * R);
+  constexpr int ITER = (TMAX + NT - 1) / NT;
+  const int total = NB * cnt;
+  float2 v[ITER][R];
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (w < total) {
+      const int t = w / NB, i = w % NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = call_load(ld, t, i + r * NB, it * R + r);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (w < total) {
+      const int i = w % NB;
+      const int k = i % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 wv = tw[r * k * TWSTEP];
+          wv.y *= -sign;
+          v[it][r] = cmul(v[it][r], wv);
+        }
+      }
+      bfly<R>(v[it], sign);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (w < total) {
+      const int t = w / NB, i = w % NB;
+      const int k = i % NS;
+      const int j = (i / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) st(t, j + r * NS, v[it][r], it * R + r);
+    }
+  }
+}
+
+template <int N, int CMAX, int NT, int S = 0, typename Load, typename Store, typename Lds>
+__device__ __forceinline__ void run_ipc(const float2* tw, float sign, int cnt, Load& first, Store& last, Lds L) {
+  constexpr int NSTAGES = num_stages(N);
+  static_assert(NSTAGES > 0, "length must factor into 2, 3, 5, 7");
+  if constexpr (NSTAGES == 1) {
+    stage_ipc<N, 0, CMAX, NT>(tw, sign, cnt, first, last);
+  } else if constexpr (S == 0) {
+    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    stage_ipc<N, 0, CMAX, NT>(tw, sign, cnt, first, st);
+    __syncthreads();
+    run_ipc<N, CMAX, NT, 1>(tw, sign, cnt, first, last, L);
+  } else if constexpr (S == NSTAGES - 1) {
+    auto ld = [&](int t, int a) { return L(t, a); };
+    stage_ipc<N, S, CMAX, NT>(tw, sign, cnt, ld, last);
+  } else {
+    auto ld = [&](int t, int a) { return L(t, a); };
+    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    stage_ipc<N, S, CMAX, NT>(tw, sign, cnt, ld, st);
+    __syncthreads();
+    run_ipc<N, CMAX, NT, S + 1>(tw, sign, cnt, first, last, L);
+  }
+}
+
 // Registers a thread holds between a forward run_ip (last stage stores by
 // slot) and the reversed-plan inverse (first stage loads by slot).
 template <int N, int CMAX, int NT, int PL = 0>
