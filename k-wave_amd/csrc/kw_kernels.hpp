@@ -102,6 +102,11 @@ struct LastArgs {
   int nin, nout;
   SpecOut outs[3];
   float scale;
+  // last-axis pass, 3-D: outputs 0 and 1 are the x and y derivatives (with
+  // kappa) of the same input; their multipliers depend on kx and ky only, so
+  // they commute with the pencil's transforms and one workgroup makes both
+  // from a single inverse transform of kappa times the spectrum
+  int pair01;
 };
 
 struct Fields3 {

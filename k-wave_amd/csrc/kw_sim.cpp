@@ -578,6 +578,12 @@ void record_non_staggered(kw_sim* s, hipStream_t st, long long t) {
   }
 }
 
+// The gradient's x and y outputs share one inverse transform in the
+// specialised last-axis pass of 3-D row-major grids (LastArgs::pair01).
+int gradient_pair(const kw_sim* s) {
+  return (s->g.dim == 3 && !s->g.kxm && !s->axisym && s->tll.nt) ? 1 : 0;
+}
+
 void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
   const Grid& g = s->g;
   const int dim = g.dim;
@@ -603,6 +609,7 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
       a.outs[c] = SpecOut{0, OP_DPOS_K, c};
     }
     a.scale = invN;
+    a.pair01 = gradient_pair(s);
     spectral(s, st, a);
     {
       Prof pr(s, st, KK_XVEL);
@@ -630,6 +637,7 @@ void step(kw_sim* s, hipStream_t st, long long t, bool final_step) {
       a.outs[c] = SpecOut{0, OP_DPOS_K, c};
     }
     a.scale = invN;
+    a.pair01 = gradient_pair(s);
     spectral(s, st, a);
   }
   Src us_src[3] = {};
@@ -1819,7 +1827,7 @@ struct SimBatch {
   LastJob* lastj[3] = {};
   YpassJob* yout[3] = {};
   int nstage = 0;
-  int stage_nin[3] = {}, stage_nout[3] = {};
+  int stage_nin[3] = {}, stage_nout[3] = {}, stage_pair[3] = {};
   bool slot_u = false, slot_p = false;
   // host copies of the tables (kept until the next rebuild)
   std::vector<char> host;
@@ -1851,6 +1859,7 @@ int step_stages(const kw_sim* s, LastArgs* st) {
       a.outs[c] = SpecOut{0, OP_DPOS_K, c};
     }
     a.scale = invN;
+    a.pair01 = gradient_pair(s);
     st[n++] = a;
   }
   {
@@ -1890,6 +1899,7 @@ void batch_build(SimBatch* b, const std::vector<kw_sim*>& sims, hipStream_t st) 
   for (int k = 0; k < b->nstage; ++k) {
     b->stage_nin[k] = stages0[k].nin;
     b->stage_nout[k] = stages0[k].nout;
+    b->stage_pair[k] = stages0[k].pair01;
   }
   const bool kxm = s0->g.kxm != 0;
   // layout
@@ -2046,7 +2056,8 @@ void batch_step(SimBatch* b, BatchSink& sk) {
       return;
     }
     if (!launch_ypass_batch(sk, g, s0->axY.n, b->yin[k], b->live, nb, b->stage_nin[k], -1.f, s0->tly) ||
-        !launch_last_batch(sk, g, s0->axL.n, b->lastj[k], b->live, nb, b->stage_nout[k], s0->tll) ||
+        !launch_last_batch(sk, g, s0->axL.n, b->lastj[k], b->live, nb, b->stage_nout[k] - b->stage_pair[k],
+                           s0->tll) ||
         !launch_ypass_batch(sk, g, s0->axY.n, b->yout[k], b->live, nb, b->stage_nout[k], 1.f, s0->tly))
       throw HipError{hipErrorInvalidConfiguration, "batched pencil pass"};
   };
