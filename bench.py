@@ -336,11 +336,13 @@ def _serve_stream(kw, reqs, arrivals, quantum: int, mode: str):
     with kw.Server(steps_per_quantum=quantum, **cfg) as srv:
         t0 = time.perf_counter()
         ids = []
-        for f, ta in zip(reqs, arrivals):
+        for k, (f, ta) in enumerate(zip(reqs, arrivals)):
             wait = t0 + ta - time.perf_counter()
             if wait > 0:
                 time.sleep(wait)
             ids.append(srv.submit(f, record=("p",)))
+            if (k + 1) % 128 == 0 and k + 1 < len(reqs):
+                log(f"serving: {mode}: {k + 1}/{len(reqs)} requests submitted")
         for r in ids:
             srv.wait(r)
         makespan = time.perf_counter() - t0
@@ -442,7 +444,10 @@ def serving_leg(kw, M: int, rates, quantum: int, accuracy_requests: int, small_r
     for r in rates:
         lam = r * capacity
         arr = list(np.cumsum(gaps / lam) - gaps[0] / lam)
-        res = {m: _serve_stream(kw, reqs, arr, quantum, m)[0] for m in SERVING_MODES}
+        res = {}
+        for m in SERVING_MODES:
+            res[m] = _serve_stream(kw, reqs, arr, quantum, m)[0]
+            log(f"serving: poisson load {r}, {m}: {res[m]}")
         out["poisson"].append({"rate_per_s": round(lam, 3), "load_vs_plain_capacity": r, **res,
                                **{f"throughput_ratio_{m}": round(res[m]["updates_per_s"] / res["plain"]["updates_per_s"], 3)
                                   for m in SERVING_MODES if m != "plain"}})

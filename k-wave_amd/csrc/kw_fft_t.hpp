@@ -130,11 +130,20 @@ struct Pad {
   __device__ static constexpr int at(int a) { return a + (a >> 4); }
 };
 
+// LDS accessors map (transform t, position a) to an element.  ptr(t, a) and
+// off(c) let a stage address R butterfly inputs or outputs from one base with
+// constant offsets (the ds_read / ds_write immediate) instead of one computed
+// address each: with the position padded every 16 (PADA), at(x + c) =
+// at(x) + at(c) holds for every x when c is a multiple of 16, and for x a
+// multiple of 16 whatever c.
 // pencils: transform index t fastest (coalesced columns), position a slowest
 template <int N, int CNT>
 struct PencilLds {
   float2* p;
   __device__ float2& operator()(int t, int a) const { return p[Pad<N>::at(a) * CNT + t]; }
+  __device__ float2* ptr(int t, int a) const { return p + Pad<N>::at(a) * CNT + t; }
+  static constexpr int off(int c) { return (c + c / 16) * CNT; }
+  static constexpr bool PADA = true;
   static constexpr int size = Pad<N>::P * CNT;
 };
 // rows: position a fastest
@@ -142,8 +151,66 @@ template <int N, int CNT>
 struct RowLds {
   float2* p;
   __device__ float2& operator()(int t, int a) const { return p[t * Pad<N>::P + Pad<N>::at(a)]; }
+  __device__ float2* ptr(int t, int a) const { return p + t * Pad<N>::P + Pad<N>::at(a); }
+  static constexpr int off(int c) { return c + c / 16; }
+  static constexpr bool PADA = true;
   static constexpr int size = Pad<N>::P * CNT;
 };
+
+// Reads and writes of an LDS accessor between two stages (the stages
+// recognise them and use base + constant offsets where the padding allows).
+template <typename Lds>
+struct LdsRead {
+  Lds L;
+  __device__ float2 operator()(int t, int a) const { return L(t, a); }
+};
+template <typename Lds>
+struct LdsWrite {
+  Lds L;
+  __device__ void operator()(int t, int a, float2 v, int) const { L(t, a) = v; }
+};
+template <typename F>
+struct LdsOf {
+  static constexpr int kind = 0;
+  static constexpr bool offs(int, bool) { return false; }
+};
+template <typename Lds>
+struct LdsOf<LdsRead<Lds>> {
+  static constexpr int kind = 1;
+  // positions x + r * step for every r from one base: step a multiple of 16
+  static constexpr bool offs(int step, bool) { return !Lds::PADA || step % 16 == 0; }
+};
+template <typename Lds>
+struct LdsOf<LdsWrite<Lds>> {
+  static constexpr int kind = 2;
+  // positions x + r * step, r < R: also exact when x is a multiple of 16
+  static constexpr bool offs(int step, bool x16) { return !Lds::PADA || step % 16 == 0 || x16; }
+};
+
+// R butterfly values at positions a0 + r * step (loads) ...
+template <int R, int STEP, bool X16, typename Load>
+__device__ __forceinline__ void load_r(Load& ld, int t, int a0, int slot0, float2* v) {
+  if constexpr (LdsOf<Load>::kind == 1 && LdsOf<Load>::offs(STEP, X16)) {
+    const float2* b = ld.L.ptr(t, a0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = b[decltype(ld.L)::off(r * STEP)];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = call_load(ld, t, a0 + r * STEP, slot0 + r);
+  }
+}
+// ... and stores
+template <int R, int STEP, bool X16, typename Store>
+__device__ __forceinline__ void store_r(Store& st, int t, int a0, int slot0, const float2* v) {
+  if constexpr (LdsOf<Store>::kind == 2 && LdsOf<Store>::offs(STEP, X16)) {
+    float2* b = st.L.ptr(t, a0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) b[decltype(st.L)::off(r * STEP)] = v[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) st(t, a0 + r * STEP, v[r], slot0 + r);
+  }
+}
 
 // One stage over `cnt` transforms (cnt <= CMAX; TFAST requires cnt == CMAX).
 // TFAST: work item w -> (t = w % CMAX, i = w / CMAX), else (t = w / NB, i = w % NB).
@@ -172,8 +239,7 @@ __device__ __forceinline__ void stage(const float2* tw, float sign, int cnt, Loa
     }
     const int k = i % NS;
     float2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = ld(t, i + r * NB);
+    load_r<R, NB, false>(ld, t, i, it * R, v);
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -184,8 +250,7 @@ __device__ __forceinline__ void stage(const float2* tw, float sign, int cnt, Loa
     }
     bfly<R>(v, sign);
     const int j = (i / NS) * NS * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) st(t, j + r * NS, v[r], it * R + r);
+    store_r<R, NS, (NS == 1 && R % 16 == 0)>(st, t, j, it * R, v);
   }
 }
 
@@ -200,16 +265,16 @@ __device__ __forceinline__ void run(const float2* tw, float sign, int cnt, Load&
   if constexpr (NSTAGES == 1) {
     stage<N, 0, CMAX, NT, TFAST>(tw, sign, cnt, first, last);
   } else if constexpr (S == 0) {
-    auto st = [&](int t, int a, float2 v, int) { A(t, a) = v; };
+    LdsWrite<Lds> st{A};
     stage<N, 0, CMAX, NT, TFAST>(tw, sign, cnt, first, st);
     __syncthreads();
     run<N, CMAX, NT, TFAST, 1>(tw, sign, cnt, first, last, A, B);
   } else if constexpr (S == NSTAGES - 1) {
-    auto ld = [&](int t, int a) { return A(t, a); };
+    LdsRead<Lds> ld{A};
     stage<N, S, CMAX, NT, TFAST>(tw, sign, cnt, ld, last);
   } else {
-    auto ld = [&](int t, int a) { return A(t, a); };
-    auto st = [&](int t, int a, float2 v, int) { B(t, a) = v; };
+    LdsRead<Lds> ld{A};
+    LdsWrite<Lds> st{B};
     stage<N, S, CMAX, NT, TFAST>(tw, sign, cnt, ld, st);
     __syncthreads();
     run<N, CMAX, NT, TFAST, S + 1>(tw, sign, cnt, first, last, B, A);
@@ -235,8 +300,7 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
     if (TMAX % NT == 0 || w < TMAX) {
       const int t = TFAST ? w % CMAX : w / NB;
       const int i = TFAST ? w / CMAX : w % NB;
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = call_load(ld, t, i + r * NB, it * R + r);
+      load_r<R, NB, false>(ld, t, i, it * R, v[it]);
     }
   }
   __syncthreads();
@@ -268,8 +332,7 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
       const int i = TFAST ? w / CMAX : w % NB;
       const int k = i % NS;
       const int j = (i / NS) * NS * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) st(t, j + r * NS, v[it][r], it * R + r);
+      store_r<R, NS, (NS == 1 && R % 16 == 0)>(st, t, j, it * R, v[it]);
     }
   }
 }
@@ -286,16 +349,16 @@ __device__ __forceinline__ void run_ip(const float2* tw, float sign, Load& first
   if constexpr (NSTAGES == 1) {
     stage_ip<N, 0, CMAX, NT, TFAST, REV, PL>(tw, sign, first, last);
   } else if constexpr (S == 0) {
-    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    LdsWrite<Lds> st{L};
     stage_ip<N, 0, CMAX, NT, TFAST, REV, PL>(tw, sign, first, st);
     __syncthreads();
     run_ip<N, CMAX, NT, TFAST, REV, PL, 1>(tw, sign, first, last, L);
   } else if constexpr (S == NSTAGES - 1) {
-    auto ld = [&](int t, int a) { return L(t, a); };
+    LdsRead<Lds> ld{L};
     stage_ip<N, S, CMAX, NT, TFAST, REV, PL>(tw, sign, ld, last);
   } else {
-    auto ld = [&](int t, int a) { return L(t, a); };
-    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    LdsRead<Lds> ld{L};
+    LdsWrite<Lds> st{L};
     stage_ip<N, S, CMAX, NT, TFAST, REV, PL>(tw, sign, ld, st);
     __syncthreads();
     run_ip<N, CMAX, NT, TFAST, REV, PL, S + 1>(tw, sign, first, last, L);
@@ -320,8 +383,7 @@ __device__ __forceinline__ void stage_ipc(const float2* tw, float sign, int cnt,
     const int w = it * NT + (int)threadIdx.x;
     if (w < total) {
       const int t = w / NB, i = w % NB;
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = call_load(ld, t, i + r * NB, it * R + r);
+      load_r<R, NB, false>(ld, t, i, it * R, v[it]);
     }
   }
   __syncthreads();
@@ -349,8 +411,7 @@ __device__ __forceinline__ void stage_ipc(const float2* tw, float sign, int cnt,
       const int t = w / NB, i = w % NB;
       const int k = i % NS;
       const int j = (i / NS) * NS * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) st(t, j + r * NS, v[it][r], it * R + r);
+      store_r<R, NS, (NS == 1 && R % 16 == 0)>(st, t, j, it * R, v[it]);
     }
   }
 }
@@ -362,16 +423,16 @@ __device__ __forceinline__ void run_ipc(const float2* tw, float sign, int cnt, L
   if constexpr (NSTAGES == 1) {
     stage_ipc<N, 0, CMAX, NT>(tw, sign, cnt, first, last);
   } else if constexpr (S == 0) {
-    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    LdsWrite<Lds> st{L};
     stage_ipc<N, 0, CMAX, NT>(tw, sign, cnt, first, st);
     __syncthreads();
     run_ipc<N, CMAX, NT, 1>(tw, sign, cnt, first, last, L);
   } else if constexpr (S == NSTAGES - 1) {
-    auto ld = [&](int t, int a) { return L(t, a); };
+    LdsRead<Lds> ld{L};
     stage_ipc<N, S, CMAX, NT>(tw, sign, cnt, ld, last);
   } else {
-    auto ld = [&](int t, int a) { return L(t, a); };
-    auto st = [&](int t, int a, float2 v, int) { L(t, a) = v; };
+    LdsRead<Lds> ld{L};
+    LdsWrite<Lds> st{L};
     stage_ipc<N, S, CMAX, NT>(tw, sign, cnt, ld, st);
     __syncthreads();
     run_ipc<N, CMAX, NT, S + 1>(tw, sign, cnt, first, last, L);
