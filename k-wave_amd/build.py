@@ -40,7 +40,7 @@ HIP_FLAGS = [
 ]
 
 SOLVER_SOURCES = (["kw_kernels.hip", "kw_sim.cpp", "kw_server.cpp", "kw_plane.hip", "kw_axisym.hip", "kw_noalloc.hip"]
-                  + [f"kw_pencil_nt{n}.hip" for n in (64, 128, 256, 512)]
+                  + [f"kw_pencil_nt{n}.hip" for n in (64, 128, 256, 512, 1024)]
                   + [f"kw_xrow_nt{n}.hip" for n in (64, 128, 256)])
 IO_SOURCES = ["kw_io.cpp"]
 

@@ -763,7 +763,8 @@ bool fast_length(int n) {
               : (nt) == 128 ? fn<128>(__VA_ARGS__) : (nt) == 256 ? fn<256>(__VA_ARGS__) : false)
 
 // pencils also have 512-thread instantiations
-#define KW_BY_NT_PENCIL(nt, fn, ...) ((nt) == 512 ? fn<512>(__VA_ARGS__) : KW_BY_NT(nt, fn, __VA_ARGS__))
+#define KW_BY_NT_PENCIL(nt, fn, ...) \
+  ((nt) == 1024 ? fn<1024>(__VA_ARGS__) : (nt) == 512 ? fn<512>(__VA_ARGS__) : KW_BY_NT(nt, fn, __VA_ARGS__))
 
 static bool ypass_fast(hipStream_t s, const Grid& g, int n, const float2* tw, float2* const* f, int nf, float sign,
                        int nt, int plan) {

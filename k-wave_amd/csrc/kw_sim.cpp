@@ -295,7 +295,7 @@ int env_nt(const char* name, int def, bool allow512 = false) {
   const char* v = std::getenv(name);
   if (!v || !*v) return def;
   const int n = std::atoi(v);
-  return (n == 0 || n == 64 || n == 128 || n == 256 || (allow512 && n == 512)) ? n : def;
+  return (n == 0 || n == 64 || n == 128 || n == 256 || (allow512 && (n == 512 || n == 1024))) ? n : def;
 }
 
 // Radix plan of a pencil length from the environment (0 or 1; plan 1 is
@@ -1124,9 +1124,11 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     // workgroup size of the length-specialised kernels (0: generic kernels)
     const int pencil_nt = env_nt("KW_PENCIL_NT", 256, true);
     // last-axis pencils of 384 points and more: 512-thread workgroups (half the
-    // registers per thread, twice the waves per CU; measured 12-21 % faster)
+    // registers per thread, twice the waves per CU; measured 12-21 % faster);
+    // 768 points: 1024 threads, whose tile is 16 pencils wide (128-byte row
+    // segments, one workgroup per CU; 768^3: 3.90 -> 3.75 ms per launch)
     const int last_nt = env_nt("KW_LAST_NT", std::getenv("KW_PENCIL_NT") ? pencil_nt
-                                                 : (s->axL.n >= 384 ? 512 : 256), true);
+                                                 : (s->axL.n == 768 ? 1024 : s->axL.n >= 384 ? 512 : 256), true);
     int row_nt = env_nt("KW_ROW_NT", g.Nx >= 256 ? 256 : 128);
     s->xl.nt = fast_length(g.Nx) ? row_nt : 0;
     s->tly.nt = (g.dim == 3 && fast_length(g.Ny)) ? pencil_nt : 0;
@@ -1659,7 +1661,7 @@ kw_status kw_sim_autotune(kw_sim* s, int trial_steps, int* row_nt, int* pencil_n
       }
       // (axisymmetric grids: the radial pass replaces the pencils)
       if (!s->g.kxm && !s->axisym && (s->tly.nt || s->tll.nt)) {
-        const std::vector<std::pair<int, int>> nts{{128, 128}, {256, 256}, {256, 512}, {512, 512}};
+        const std::vector<std::pair<int, int>> nts{{128, 128}, {256, 256}, {256, 512}, {512, 512}, {256, 1024}};
         const bool py = s->g.dim == 3 && s->axY.n >= 160, pl = s->axL.n >= 160;
         for (const auto& pe : nts)
           for (int a = 0; a <= (py ? 1 : 0); ++a)
