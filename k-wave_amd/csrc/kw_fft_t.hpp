@@ -285,7 +285,9 @@ __device__ __forceinline__ void run(const float2* tw, float sign, int cnt, Load&
 // work item's inputs are loaded into registers, then a barrier, then the
 // butterflies and stores (a Stockham stage writes other positions than it
 // reads, so without the barrier a store could overwrite an unread input).
-template <int N, int S, int CMAX, int NT, bool TFAST, bool REV = false, int PL = 0, typename Load, typename Store>
+// BAR false drops that barrier where no such hazard exists (see run_ip's flags).
+template <int N, int S, int CMAX, int NT, bool TFAST, bool REV = false, int PL = 0, bool BAR = true, typename Load,
+          typename Store>
 __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld, Store& st) {
   constexpr int R = plan_radix<REV>(N, S, PL);
   constexpr int NS = plan_ns<REV>(N, S, PL);
@@ -303,7 +305,7 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
       load_r<R, NB, false>(ld, t, i, it * R, v[it]);
     }
   }
-  __syncthreads();
+  if constexpr (BAR) __syncthreads();
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
     const int w = it * NT + (int)threadIdx.x;
@@ -337,38 +339,45 @@ __device__ __forceinline__ void stage_ip(const float2* tw, float sign, Load& ld,
   }
 }
 
+// Caller guarantees for run_ip (flags FL): kFree0 -- no thread reads or
+// writes L when the transform starts, so the first stage may store into L
+// without waiting; kNoLdsLast -- `last` does not write LDS, so the final stage
+// needs no barrier between its LDS reads and its stores.  Without them every
+// stage waits for all reads of the buffer before its stores overwrite it.
+constexpr unsigned kFree0 = 1, kNoLdsLast = 2;
+
 // Full transform of all CMAX transforms in one LDS buffer L (in place):
 // stage 0 loads through `first`, the last stage stores through `last`, the
 // others read and write L.  Stages are separated by barriers; there is no
 // trailing barrier.
-template <int N, int CMAX, int NT, bool TFAST, bool REV = false, int PL = 0, int S = 0, typename Load,
-          typename Store, typename Lds>
+template <int N, int CMAX, int NT, bool TFAST, bool REV = false, int PL = 0, unsigned FL = 0, int S = 0,
+          typename Load, typename Store, typename Lds>
 __device__ __forceinline__ void run_ip(const float2* tw, float sign, Load& first, Store& last, Lds L) {
   constexpr int NSTAGES = num_stages(N, PL);
   static_assert(NSTAGES > 0, "length must factor into 2, 3, 5, 7");
   if constexpr (NSTAGES == 1) {
-    stage_ip<N, 0, CMAX, NT, TFAST, REV, PL>(tw, sign, first, last);
+    stage_ip<N, 0, CMAX, NT, TFAST, REV, PL, !(FL & (kFree0 | kNoLdsLast))>(tw, sign, first, last);
   } else if constexpr (S == 0) {
     LdsWrite<Lds> st{L};
-    stage_ip<N, 0, CMAX, NT, TFAST, REV, PL>(tw, sign, first, st);
+    stage_ip<N, 0, CMAX, NT, TFAST, REV, PL, !(FL & kFree0)>(tw, sign, first, st);
     __syncthreads();
-    run_ip<N, CMAX, NT, TFAST, REV, PL, 1>(tw, sign, first, last, L);
+    run_ip<N, CMAX, NT, TFAST, REV, PL, FL, 1>(tw, sign, first, last, L);
   } else if constexpr (S == NSTAGES - 1) {
     LdsRead<Lds> ld{L};
-    stage_ip<N, S, CMAX, NT, TFAST, REV, PL>(tw, sign, ld, last);
+    stage_ip<N, S, CMAX, NT, TFAST, REV, PL, !(FL & kNoLdsLast)>(tw, sign, ld, last);
   } else {
     LdsRead<Lds> ld{L};
     LdsWrite<Lds> st{L};
     stage_ip<N, S, CMAX, NT, TFAST, REV, PL>(tw, sign, ld, st);
     __syncthreads();
-    run_ip<N, CMAX, NT, TFAST, REV, PL, S + 1>(tw, sign, first, last, L);
+    run_ip<N, CMAX, NT, TFAST, REV, PL, FL, S + 1>(tw, sign, first, last, L);
   }
 }
 
 // In-place stage and transform over a runtime number of transforms cnt <=
 // CMAX, work items transform-major (t = w / NB), as stage()/run() distribute
 // them: the same slot numbering, so last_pos() maps the last stage's slots.
-template <int N, int S, int CMAX, int NT, typename Load, typename Store>
+template <int N, int S, int CMAX, int NT, bool BAR = true, typename Load, typename Store>
 __device__ __forceinline__ void stage_ipc(const float2* tw, float sign, int cnt, Load& ld, Store& st) {
   constexpr int R = radix_of(N, S);
   constexpr int NS = ns_of(N, S);
@@ -386,7 +395,7 @@ __device__ __forceinline__ void stage_ipc(const float2* tw, float sign, int cnt,
       load_r<R, NB, false>(ld, t, i, it * R, v[it]);
     }
   }
-  __syncthreads();
+  if constexpr (BAR) __syncthreads();
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
     const int w = it * NT + (int)threadIdx.x;
@@ -416,26 +425,26 @@ __device__ __forceinline__ void stage_ipc(const float2* tw, float sign, int cnt,
   }
 }
 
-template <int N, int CMAX, int NT, int S = 0, typename Load, typename Store, typename Lds>
+template <int N, int CMAX, int NT, unsigned FL = 0, int S = 0, typename Load, typename Store, typename Lds>
 __device__ __forceinline__ void run_ipc(const float2* tw, float sign, int cnt, Load& first, Store& last, Lds L) {
   constexpr int NSTAGES = num_stages(N);
   static_assert(NSTAGES > 0, "length must factor into 2, 3, 5, 7");
   if constexpr (NSTAGES == 1) {
-    stage_ipc<N, 0, CMAX, NT>(tw, sign, cnt, first, last);
+    stage_ipc<N, 0, CMAX, NT, !(FL & (kFree0 | kNoLdsLast))>(tw, sign, cnt, first, last);
   } else if constexpr (S == 0) {
     LdsWrite<Lds> st{L};
-    stage_ipc<N, 0, CMAX, NT>(tw, sign, cnt, first, st);
+    stage_ipc<N, 0, CMAX, NT, !(FL & kFree0)>(tw, sign, cnt, first, st);
     __syncthreads();
-    run_ipc<N, CMAX, NT, 1>(tw, sign, cnt, first, last, L);
+    run_ipc<N, CMAX, NT, FL, 1>(tw, sign, cnt, first, last, L);
   } else if constexpr (S == NSTAGES - 1) {
     LdsRead<Lds> ld{L};
-    stage_ipc<N, S, CMAX, NT>(tw, sign, cnt, ld, last);
+    stage_ipc<N, S, CMAX, NT, !(FL & kNoLdsLast)>(tw, sign, cnt, ld, last);
   } else {
     LdsRead<Lds> ld{L};
     LdsWrite<Lds> st{L};
     stage_ipc<N, S, CMAX, NT>(tw, sign, cnt, ld, st);
     __syncthreads();
-    run_ipc<N, CMAX, NT, S + 1>(tw, sign, cnt, first, last, L);
+    run_ipc<N, CMAX, NT, FL, S + 1>(tw, sign, cnt, first, last, L);
   }
 }
 
