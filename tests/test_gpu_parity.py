@@ -120,14 +120,16 @@ def test_p0_equals_two_clicks_gpu(kw, oracle):
     assert cases.rel_l2(a, oracle.run(f1)["p"]) < 1e-4
 
 
-@pytest.mark.parametrize("shape", [(64, 48, 96), (48, 64, 32), (40, 32, 192)])
+@pytest.mark.parametrize("shape", [(64, 48, 96), (48, 64, 32), (40, 32, 192), (32, 48, 768), (48, 768, 32)])
 def test_3d_specialised_pencil_lengths(kw, oracle, shape):
-    """Grids whose y/z lengths have compiled pencil kernels (others use the generic ones)."""
+    """Grids whose y/z lengths have compiled pencil kernels (others use the generic ones);
+    a 768-point last axis runs the 1024-thread, 16-pencil tiles, a 768-point y axis
+    the 8-pencil ones."""
     f = cases.case_3d(N=shape, Nt=30, nonlinear=True, source="p", pml=4)
     compare(kw, oracle, f, ("p", "p_max", "u_final"), f"3d_pencil_{shape}")
 
 
-@pytest.mark.parametrize("shape", [(96, 160), (128, 128), (64, 256)])
+@pytest.mark.parametrize("shape", [(96, 160), (128, 128), (64, 256), (64, 768)])
 def test_2d_specialised_last_axis(kw, oracle, shape):
     f = cases.case_2d_absorbing_psource(N=shape, Nt=60)
     compare(kw, oracle, f, ("p", "p_rms"), f"2d_last_{shape}")
