@@ -93,3 +93,13 @@ def test_c5_grid_batched_equals_alone_and_oracle(kw, oracle, grid):
     mx = float(np.max(np.abs(outs[0]["p"].astype(np.float64) - ref)))
     print(f"    vs FP64 oracle: rel L2 {err:.3e}, max abs {mx:.3e} (peak {float(np.max(np.abs(ref))):.3e})", flush=True)
     assert err <= 1e-4
+
+
+def test_batched_768_point_last_axis_equals_alone(kw):
+    """A 768-point last axis runs 1024-thread, 16-pencil tiles (the CU's whole
+    LDS) in the batched kernels too: two requests per launch, bit-identical to
+    each run alone."""
+    items = [(cases.case_3d(N=(32, 48, 768), Nt=14, source="p", pml=4, seed=s), ("p", "p_max"), 1) for s in (3, 4)]
+    print("[batching] 2 requests of 32x48x768, quantum 5", flush=True)
+    outs = _serve(kw, items, max_active=4, steps_per_quantum=5)
+    _check_equal(kw, items, outs)
