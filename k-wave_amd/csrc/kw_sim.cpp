@@ -1342,7 +1342,12 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     s->nrec_total = s->Nt - s->start + 1;
     const size_t nsz = std::max(s->ns, 1);
     const uint64_t rf = s->rec;
-    if (rf & KW_REC_P) s->p_raw = s->alloc<float>(nsz * (size_t)s->nrec_total);
+    // time series start zeroed: a fetch before the last step reads zeros for
+    // the steps still to come, not whatever the memory held
+    if (rf & KW_REC_P) {
+      s->p_raw = s->alloc<float>(nsz * (size_t)s->nrec_total);
+      s->zero(s->p_raw, nsz * (size_t)s->nrec_total * 4);
+    }
     if (rf & KW_REC_P_MAX) s->p_max = s->alloc<float>(nsz);
     if (rf & KW_REC_P_MIN) s->p_min = s->alloc<float>(nsz);
     if (rf & KW_REC_P_RMS) s->p_rms = s->alloc<double>(2 * nsz);
@@ -1350,13 +1355,19 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     if (rf & KW_REC_P_MIN_ALL) s->p_min_all = s->alloc<float>((size_t)N);
     if (rf & KW_REC_P_FINAL) s->p_final = s->alloc<float>((size_t)N);
     for (int a = 0; a < g.dim; ++a) {
-      if (rf & KW_REC_U) s->u_raw[a] = s->alloc<float>(nsz * (size_t)s->nrec_total);
+      if (rf & KW_REC_U) {
+        s->u_raw[a] = s->alloc<float>(nsz * (size_t)s->nrec_total);
+        s->zero(s->u_raw[a], nsz * (size_t)s->nrec_total * 4);
+      }
       if (rf & KW_REC_U_MAX) s->u_max[a] = s->alloc<float>(nsz);
       if (rf & KW_REC_U_MIN) s->u_min[a] = s->alloc<float>(nsz);
       if (rf & KW_REC_U_RMS) s->u_rms[a] = s->alloc<double>(2 * nsz);
       if (rf & KW_REC_U_MAX_ALL) s->u_max_all[a] = s->alloc<float>((size_t)N);
       if (rf & KW_REC_U_MIN_ALL) s->u_min_all[a] = s->alloc<float>((size_t)N);
-      if (rf & KW_REC_U_NON_STAGGERED) s->u_ns[a] = s->alloc<float>(nsz * (size_t)s->nrec_total);
+      if (rf & KW_REC_U_NON_STAGGERED) {
+        s->u_ns[a] = s->alloc<float>(nsz * (size_t)s->nrec_total);
+        s->zero(s->u_ns[a], nsz * (size_t)s->nrec_total * 4);
+      }
     }
 
     // sources (point lists share the row CSR construction)

@@ -43,3 +43,26 @@ def test_no_sensor_points_records_p_final(kw, oracle):
     ref = oracle.run(f, record=("p_final", "p_max_all"))
     for k in ("p_final", "p_max_all"):
         assert cases.rel_l2(got[k], ref[k]) <= 1e-4, k
+
+
+def test_time_series_start_zeroed(kw):
+    """Recorded time series are zero-filled at creation, so a fetch before the
+    last step returns zeros for the steps still to come -- also when the memory
+    was just released by another simulation of the same size."""
+    f = cases.case_3d(N=(32, 28, 24), Nt=30, source="p")
+    rec = ("p", "u")
+    names = ("p", "ux", "uy", "uz")
+    print("[gpu memory] partial runs: rows of steps not yet run read zero", flush=True)
+    with kw.Simulation(f, record=rec) as sim:
+        sim.run(30)
+        sim.sync()
+        full = {k: sim.fetch(k) for k in names}
+    for _ in range(2):
+        with kw.Simulation(f, record=rec) as sim:
+            sim.run(20)
+            sim.sync()
+            for k in names:
+                got = sim.fetch(k)  # (recorded steps, sensor points)
+                assert got.shape == full[k].shape and got.shape[0] == 30, (k, got.shape)
+                assert np.array_equal(got[:20], full[k][:20]), k
+                assert not np.any(got[20:]), k
