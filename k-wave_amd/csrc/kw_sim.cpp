@@ -1129,7 +1129,9 @@ static kw_status create_impl(const kw_sim_desc* d, char* arena, size_t arena_cap
     // segments, one workgroup per CU; 768^3: 3.90 -> 3.75 ms per launch)
     const int last_nt = env_nt("KW_LAST_NT", std::getenv("KW_PENCIL_NT") ? pencil_nt
                                                  : (s->axL.n == 768 ? 1024 : s->axL.n >= 384 ? 512 : 256), true);
-    int row_nt = env_nt("KW_ROW_NT", g.Nx >= 256 ? 256 : 128);
+    // 256-thread rows from 256 points up and at 160 (whose halved row pairs
+    // leave 256 threads five slots each: 160x160x192 step 0.547 -> 0.507 ms)
+    int row_nt = env_nt("KW_ROW_NT", (g.Nx >= 256 || g.Nx == 160) ? 256 : 128);
     s->xl.nt = fast_length(g.Nx) ? row_nt : 0;
     s->tly.nt = (g.dim == 3 && fast_length(g.Ny)) ? pencil_nt : 0;
     s->tll.nt = fast_length(s->axL.n) ? last_nt : 0;
